@@ -1,0 +1,137 @@
+"""ctypes binding of libhermespy_amd.so (the C ABI declared in include/hermespy_amd.h).
+
+torch is imported first so that the library binds to the HIP runtime torch
+already loaded (both carry the soname libamdhip64.so.7): device pointers and
+streams from torch are then valid in the library.  There is no fallback: if
+the library is missing, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede the library load, see module docstring)
+
+LIB_PATH = Path(__file__).resolve().parent / "_native" / "libhermespy_amd.so"
+
+HPX_OK = 0
+HPX_ERR_INVALID = -1
+HPX_ERR_UNSUPPORTED = -2
+HPX_ERR_DEVICE = -3
+
+HPX_DETECT_ZF = 0
+HPX_DETECT_MMSE = 1
+
+STREAM_REALIZATION = 0x000
+STREAM_BITS = 0x100
+STREAM_NOISE = 0x200
+
+c_int = ctypes.c_int
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_u32 = ctypes.c_uint32
+c_u64 = ctypes.c_uint64
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+
+
+class HpxLink(ctypes.Structure):
+    """Mirror of ``hpx_link``."""
+
+    _fields_ = [
+        ("num_rx", c_i32), ("num_tx", c_i32),
+        ("num_subcarriers", c_i32), ("words", c_i32), ("prefix", c_i32),
+        ("modulation_order", c_i32),
+        ("num_paths", c_i32), ("num_sinusoids", c_i32),
+        ("max_delay", c_i32),
+        ("detector", c_i32),
+        ("dc_suppression", c_i32),
+        ("bandwidth", c_dbl), ("doppler", c_dbl), ("los_doppler", c_dbl), ("gain", c_dbl),
+        ("device_power", c_dbl),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "hpx_abi_version": (c_int, []),
+    "hpx_last_error": (c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "hpx_fading_taps": (c_int, [c_vp, c_int, c_int, c_i64, c_vp, c_vp]),
+    "hpx_tdl_propagate": (c_int, [c_vp, c_int, c_i64, c_i64, c_vp, c_int, c_int, c_dbl, c_vp, c_i32, c_vp, c_int,
+                                  c_vp, c_i64, c_vp]),
+    "hpx_awgn": (c_int, [c_vp, c_int, c_i64, c_i64, c_u64, c_u32, c_dbl, c_vp]),
+    "hpx_map_symbols": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp]),
+    "hpx_demap_hard": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp]),
+    "hpx_constellation_host": (c_int, [c_int, c_int, c_vp]),
+    "hpx_demap_hard_host": (c_int, [c_vp, c_i64, c_int, c_int, c_vp]),
+    "hpx_ofdm_modulate": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_i64, c_vp]),
+    "hpx_ofdm_demodulate": (c_int, [c_vp, c_int, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp]),
+    "hpx_ideal_csi": (c_int, [c_vp, c_int, c_int, c_vp, c_i32, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_vp,
+                              c_vp]),
+    "hpx_equalize": (c_int, [c_vp, c_vp, c_int, c_int, c_i64, c_int, c_dbl, c_vp, c_vp]),
+    "hpx_count_errors": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "hpx_run_batch": (c_int, [ctypes.POINTER(HpxLink), c_vp, c_vp, c_vp, c_vp, c_u64, c_i64, c_i64, c_vp, c_vp, c_vp,
+                              c_vp]),
+    "hpx_run_batch_lds_bytes": (c_i64, [ctypes.POINTER(HpxLink)]),
+}
+
+_lib: ctypes.CDLL | None = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the library (once).  Raises RuntimeError when it was not built."""
+
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get("HPX_LIBRARY", str(LIB_PATH)))
+    if not path.exists():
+        raise RuntimeError(f"libhermespy_amd.so not found at {path}: run `python -m hermespy_amd.build` first")
+    lib = ctypes.CDLL(str(path))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.hpx_abi_version() != 1:
+        raise RuntimeError("libhermespy_amd ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(512)
+    load().hpx_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int) -> None:
+    """Raise the reference's exception class for a failed call."""
+
+    if status == HPX_OK:
+        return
+    msg = last_error()
+    if status in (HPX_ERR_INVALID, HPX_ERR_UNSUPPORTED):
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(t) -> int | None:
+    """Device (or host) address of a tensor, None for None."""
+
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    """Raw hipStream_t of torch's current stream."""
+
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_device(t, name: str) -> None:
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")

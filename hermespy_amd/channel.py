@@ -1,0 +1,307 @@
+"""Statistical multipath fading channel with the reference's API, propagating on the GPU.
+
+Mirrors ``hermespy/channel/fading/fading.py``: ``CustomAntennaCorrelation``
+(:92-133), ``MultipathFadingChannel`` (:585-910), ``MultipathFadingRealization``
+(:425-515) and ``MultipathFadingSample`` (:147-406).  Realisations are drawn on
+the host exactly as the reference draws them -- one ``los_angle`` uniform at
+construction (:713) and ``standard_normal(100 + 2L + 2LN)`` per ``realize()``
+from the channel's NumPy generator (``hermespy/channel/consistent.py:410-432``)
+-- so a seeded channel yields the reference's realisation.  Taps, propagation
+and the dense impulse response run in HIP through libhermespy_amd.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+from scipy.special import ndtr
+
+from . import _lib
+
+
+class AntennaCorrelation:
+    """Base class of antenna correlation models (``fading.py:44-89``)."""
+
+    def sample_covariance(self, num_antennas: int) -> np.ndarray:  # pragma: no cover - interface
+        raise NotImplementedError
+
+
+class CustomAntennaCorrelation(AntennaCorrelation):
+    """Fixed Hermitian positive definite covariance (``fading.py:92-133``)."""
+
+    def __init__(self, covariance: np.ndarray) -> None:
+        self.covariance = covariance
+
+    @property
+    def covariance(self) -> np.ndarray:
+        return self.__covariance
+
+    @covariance.setter
+    def covariance(self, value: np.ndarray) -> None:
+        value = np.asarray(value)
+        if value.ndim != 2 or not np.allclose(value, value.T.conj()):
+            raise ValueError("Antenna correlation must be a hermitian matrix")
+        if np.any(np.linalg.eigvals(value) <= 0.0):
+            raise ValueError("Antenna correlation matrix must be positive definite")
+        self.__covariance = value
+
+    def sample_covariance(self, num_antennas: int) -> np.ndarray:
+        if self.__covariance.shape[0] < num_antennas:
+            raise ValueError("Antenna correlation matrix does not match the number of antennas")
+        return self.__covariance[:num_antennas, :num_antennas]
+
+
+def _as_device_signal(signal, device) -> torch.Tensor:
+    if isinstance(signal, torch.Tensor):
+        t = signal.to(device=device, dtype=torch.complex64)
+    else:
+        t = torch.as_tensor(np.asarray(signal, dtype=np.complex64), device=device)
+    if t.ndim == 1:
+        t = t[None, :]
+    return t.contiguous()
+
+
+class MultipathFadingSample:
+    """One sampled link of a fading realisation (``fading.py:147-406``)."""
+
+    def __init__(self, power_profile, delay_profile, los_angles, nlos_angles, los_phases, nlos_phases, los_gains,
+                 nlos_gains, los_doppler, nlos_doppler, spatial_response, gain, bandwidth, device="cuda") -> None:
+        self.power_profile = np.asarray(power_profile, dtype=np.float64)
+        self.delay_profile = np.asarray(delay_profile, dtype=np.float64)
+        self.los_angles = np.asarray(los_angles, dtype=np.float64)
+        self.nlos_angles = np.asarray(nlos_angles, dtype=np.float64)
+        self.los_phases = np.asarray(los_phases, dtype=np.float64)
+        self.nlos_phases = np.asarray(nlos_phases, dtype=np.float64)
+        self.los_gains = np.asarray(los_gains, dtype=np.float64)
+        self.nlos_gains = np.asarray(nlos_gains, dtype=np.float64)
+        self.los_doppler = float(los_doppler)
+        self.nlos_doppler = float(nlos_doppler)
+        self.spatial_response = np.asarray(spatial_response, dtype=np.complex128)
+        self.gain = float(gain)
+        self.bandwidth = float(bandwidth)
+        self.device = torch.device(device)
+        self._dev_cache: dict | None = None
+
+    # ------------------------------------------------------------ properties
+    @property
+    def num_paths(self) -> int:
+        return int(self.delay_profile.size)
+
+    @property
+    def num_sinusoids(self) -> int:
+        return int(self.nlos_angles.shape[1])
+
+    @property
+    def num_receive_antennas(self) -> int:
+        return int(self.spatial_response.shape[0])
+
+    @property
+    def num_transmit_antennas(self) -> int:
+        return int(self.spatial_response.shape[1])
+
+    @property
+    def expected_energy_scale(self) -> float:
+        """``fading.py:289-291``."""
+
+        return self.gain * float(np.sum(self.power_profile))
+
+    @property
+    def delay_samples(self) -> np.ndarray:
+        return np.rint(self.delay_profile * self.bandwidth).astype(np.int32)  # fading.py:297
+
+    @property
+    def max_delay_samples(self) -> int:
+        return int(round(float(self.delay_profile.max()) * self.bandwidth))  # fading.py:372
+
+    def sos_table(self) -> np.ndarray:
+        """[L][N + 1][3] (rate rad/sample, phase, amplitude): the NLOS sinusoids of
+        ``fading.py:326-336`` followed by the specular term of :339, scaled as :342."""
+
+        L, N = self.num_paths, self.num_sinusoids
+        n = 1 + np.arange(N)
+        scale = np.sqrt(self.gain * self.power_profile)
+        t = np.empty((L, N + 1, 3), dtype=np.float64)
+        t[:, :N, 0] = self.nlos_doppler * np.cos((2 * np.pi * n[None, :] + self.nlos_angles) / N) / self.bandwidth
+        t[:, :N, 1] = self.nlos_phases
+        t[:, :N, 2] = (self.nlos_gains * scale)[:, None]
+        t[:, N, 0] = self.los_doppler * np.cos(self.los_angles) / self.bandwidth
+        t[:, N, 1] = self.los_phases
+        t[:, N, 2] = self.los_gains * scale
+        return t
+
+    def _device_tables(self):
+        if self._dev_cache is None:
+            tab = self.sos_table()
+            self._dev_cache = {
+                "sos": torch.as_tensor(tab, device=self.device).contiguous(),
+                "delays": torch.as_tensor(self.delay_samples, device=self.device),
+                "spatial": torch.as_tensor(self.spatial_response.astype(np.complex64), device=self.device),
+                "max_rate": float(np.max(np.abs(tab[:, :, 0]))) if tab.size else 0.0,
+            }
+        return self._dev_cache
+
+    # ------------------------------------------------------------ kernels
+    def taps(self, num_samples: int) -> torch.Tensor:
+        """All path impulses (L, num_samples) complex64, direct SoS evaluation (``fading.py:293-343``)."""
+
+        d = self._device_tables()
+        out = torch.empty((self.num_paths, int(num_samples)), dtype=torch.complex64, device=self.device)
+        _lib.check(_lib.load().hpx_fading_taps(_lib.ptr(d["sos"]), self.num_paths, self.num_sinusoids + 1,
+                                               int(num_samples), _lib.ptr(out), _lib.stream_handle()))
+        return out
+
+    def propagate(self, signal) -> torch.Tensor:
+        """``_propagate`` (``fading.py:371-406``): (Ntx, M) -> (Nrx, M + max delay), complex64 on the device."""
+
+        x = _as_device_signal(signal, self.device)
+        if x.shape[0] != self.num_transmit_antennas:
+            raise ValueError(f"Number of signal streams ({x.shape[0]}) does not match the number of transmit "
+                             f"antennas ({self.num_transmit_antennas})")
+        d = self._device_tables()
+        m = int(x.shape[1])
+        dmax = self.max_delay_samples
+        y = torch.empty((self.num_receive_antennas, m + dmax), dtype=torch.complex64, device=self.device)
+        if y.numel() == 0:
+            return y
+        _lib.check(_lib.load().hpx_tdl_propagate(
+            _lib.ptr(x), self.num_transmit_antennas, m, m, _lib.ptr(d["sos"]), self.num_paths, self.num_sinusoids + 1,
+            d["max_rate"], _lib.ptr(d["delays"]), dmax, _lib.ptr(d["spatial"]), self.num_receive_antennas,
+            _lib.ptr(y), m + dmax, _lib.stream_handle()))
+        return y
+
+    def state(self, num_samples: int, max_num_taps: int) -> torch.Tensor:
+        """Dense impulse response (Nrx, Ntx, num_samples, taps) (``fading.py:345-369``)."""
+
+        num_taps = min(1 + round(float(self.delay_profile.max()) * self.bandwidth), int(max_num_taps))
+        h = self.taps(num_samples)
+        siso = torch.zeros((int(num_samples), num_taps), dtype=torch.complex64, device=self.device)
+        for p, dl in enumerate(self.delay_samples):
+            if dl > num_taps:
+                continue
+            siso[:, dl] += h[p]
+        s = torch.as_tensor(self.spatial_response.astype(np.complex64), device=self.device)
+        return s[:, :, None, None] * siso[None, None, :, :]
+
+
+class MultipathFadingRealization:
+    """Realisation of a fading channel (``fading.py:425-515``)."""
+
+    def __init__(self, scalars: np.ndarray, channel: "MultipathFadingChannel") -> None:
+        self.scalars = np.asarray(scalars, dtype=np.float64)
+        self.channel = channel
+
+    def sample(self, num_transmit_antennas: int, num_receive_antennas: int, bandwidth: float,
+               device="cuda") -> MultipathFadingSample:
+        """``_sample`` (``fading.py:468-515``) for a link of the given antenna counts and sampling rate."""
+
+        ch = self.channel
+        L, N = ch.num_resolvable_paths, ch.num_sinusoids
+        if num_receive_antennas > 10 or num_transmit_antennas > 10:
+            raise ValueError("the fading spatial response is drawn on a 10 x 10 grid (at most 10 antennas)")
+        u = ndtr(self.scalars)  # ConsistentUniform.sample, consistent.py:485
+        o = 0
+        corr = u[o:o + 100].reshape(10, 10); o += 100
+        los_u = u[o:o + L]; o += L
+        nlos_ang_u = u[o:o + L * N].reshape(L, N); o += L * N
+        los_ph_u = u[o:o + L]; o += L
+        nlos_ph_u = u[o:o + L * N].reshape(L, N)
+        spatial = np.exp(2j * np.pi * corr)[:num_receive_antennas, :num_transmit_antennas]
+        if ch.antenna_correlation is not None:
+            spatial = (ch.antenna_correlation.sample_covariance(num_receive_antennas) @ spatial
+                       @ ch.antenna_correlation.sample_covariance(num_transmit_antennas))
+        return MultipathFadingSample(ch.power_profile, ch.delays, 2 * np.pi * los_u, -np.pi + 2 * np.pi * nlos_ang_u,
+                                     -np.pi + 2 * np.pi * los_ph_u, -np.pi + 2 * np.pi * nlos_ph_u, ch.los_gains,
+                                     ch.nlos_gains, ch.los_doppler_frequency, ch.doppler_frequency, spatial, ch.gain,
+                                     bandwidth, device)
+
+
+class MultipathFadingChannel:
+    """Statistical multipath fading channel (``fading.py:585-910``)."""
+
+    _DEFAULT_NUM_SINUSOIDS = 20
+
+    def __init__(self, delays, power_profile, rice_factors, correlation_distance: float = float("inf"),
+                 num_sinusoids: int = _DEFAULT_NUM_SINUSOIDS, los_angle: float | None = None,
+                 doppler_frequency: float = 0.0, los_doppler_frequency: float | None = None,
+                 antenna_correlation: AntennaCorrelation | None = None, gain: float = 1.0,
+                 seed: int | None = None) -> None:
+        d = np.array(delays) if isinstance(delays, list) else np.asarray(delays)
+        p = np.array(power_profile) if isinstance(power_profile, list) else np.asarray(power_profile)
+        k = np.array(rice_factors) if isinstance(rice_factors, list) else np.asarray(rice_factors)
+        if d.ndim != 1 or p.ndim != 1 or k.ndim != 1:
+            raise ValueError("Delays, power profile and rice factors must be vectors")
+        if len(d) < 1:
+            raise ValueError("Configuration must contain at least one delay tap")
+        if len(d) != len(p) or len(p) != len(k):
+            raise ValueError("Delays, power profile and rice factor vectors must be of equal length")
+        if np.any(d < 0.0):
+            raise ValueError("Delays must be greater or equal to zero")
+        if np.any(p < 0.0):
+            raise ValueError("Power profile factors must be greater or equal to zero")
+        if np.any(k < 0.0):
+            raise ValueError("Rice factors must be greater or equal to zero")
+        if correlation_distance < 0:
+            raise ValueError("Correlation distance must be greater or equal to zero")
+        if correlation_distance != float("inf"):
+            raise ValueError("only spatially static channels (correlation_distance = inf) run on this path")
+        self.gain = float(gain)
+        self.seed = seed  # RandomNode: a fresh generator, core/random_node.py
+        order = np.argsort(d)
+        self.delays = d[order].astype(np.float64)
+        self.power_profile = p[order].astype(np.float64)
+        self.rice_factors = k[order].astype(np.float64)
+        self.num_sinusoids = int(num_sinusoids)
+        self.los_angle = self._rng.uniform(-np.pi, np.pi) if los_angle is None else los_angle  # fading.py:713
+        self.doppler_frequency = float(doppler_frequency)
+        self.__los_doppler = los_doppler_frequency
+        self.antenna_correlation = antenna_correlation
+        self.correlation_distance = correlation_distance
+        inf = np.isposinf(self.rice_factors)
+        self.los_gains = np.ones(self.delays.size)
+        self.los_gains[~inf] = np.sqrt(self.rice_factors[~inf] / (1 + self.rice_factors[~inf]))
+        self.nlos_gains = np.zeros(self.delays.size)
+        self.nlos_gains[~inf] = np.sqrt(1 / ((1 + self.rice_factors[~inf]) * self.num_sinusoids))
+        self.realization: MultipathFadingRealization | None = None
+
+    @property
+    def seed(self) -> int | None:
+        return self.__seed
+
+    @seed.setter
+    def seed(self, value: int | None) -> None:
+        self.__seed = value
+        self._rng = np.random.default_rng(value)
+
+    @property
+    def num_resolvable_paths(self) -> int:
+        return int(self.delays.size)
+
+    @property
+    def max_delay(self) -> float:
+        return float(self.delays.max())
+
+    @property
+    def los_doppler_frequency(self) -> float:
+        return self.doppler_frequency if self.__los_doppler is None else float(self.__los_doppler)
+
+    @property
+    def num_scalars(self) -> int:
+        L, N = self.num_resolvable_paths, self.num_sinusoids
+        return 100 + 2 * L + 2 * L * N
+
+    def realize(self, cache: bool = True) -> MultipathFadingRealization:
+        """``Channel.realize`` -> ``_realize`` (``channel.py:877-898``, ``fading.py:893-910``)."""
+
+        real = MultipathFadingRealization(self._rng.standard_normal(self.num_scalars), self)
+        if cache:
+            self.realization = real
+        return real
+
+    def propagate(self, signal, num_receive_antennas: int, bandwidth: float, device="cuda") -> torch.Tensor:
+        """``Channel.propagate`` (``channel.py:910-973``): realise, sample, propagate."""
+
+        x = _as_device_signal(signal, device)
+        sample = self.realize().sample(int(x.shape[0]), num_receive_antennas, bandwidth, device)
+        return sample.propagate(x)

@@ -1,0 +1,511 @@
+// Fused Monte Carlo link drops: one workgroup simulates one drop end to end.
+//
+// Per drop (prologue):
+//   Philox normals -> Phi -> spatial response S = R_rx exp(2 pi j U) R_tx and
+//   the SoS table (fading.py:468-499, consistent.py:410-485), Taylor
+//   coefficients of every path over the frame, per-word tap means (ideal CSI),
+//   twiddles and the constellation table.
+// Per OFDM word w (main loop; nothing but the error count leaves the CU):
+//   bits -> 3GPP map -> IFFT -> CP                      (modem.py:797-888, ofdm.py:97-130)
+//   time-variant TDL + S over the word's N data samples (fading.py:371-406)
+//     + AWGN                                             (rf/noise/model.py:143-160)
+//   FFT -> ideal CSI -> ZF/MMSE -> hard demap -> XOR-popcount against the bits
+//     (ofdm.py:132-161, channel_estimation.py:215-295, waveform.py:235-250,
+//      psk_qam_mapping.py:171-259, evaluators.py:239-258)
+// Only the data windows the receiver keeps are propagated: the cyclic prefixes
+// and the delay tail are discarded by the reference's demodulation
+// (waveform.py:567-614) and default synchronisation (waveform.py:90-99).
+#include <math.h>
+
+#include <algorithm>
+
+#include "hpx_common.hpp"
+#include "hpx_detect.hpp"
+#include "hpx_fading.hpp"
+#include "hpx_fft.hpp"
+#include "hpx_mapping.hpp"
+
+namespace hpx {
+int taylor_log2_block(double max_rate, int terms, int max_log2);
+int ilog2_exact(int n);
+}  // namespace hpx
+
+using namespace hpx;
+
+namespace {
+
+constexpr int kBatchMaxThreads = 512;
+constexpr int kTailPerThread = 4;  // registers holding the previous word's tail
+
+struct BatchArgs {
+  // link
+  int n, log2n, cp, words, bps, order, detector, dc;
+  int paths, sinusoids, terms, nnorm;
+  int log2b, nb, dx, maxd, keep_below, xs, bits_words;
+  int64_t frame_len;
+  double bandwidth, doppler, los_doppler, tx_scale;
+  Modulation mod;
+  // arrays
+  const int32_t* delays;
+  const double* path_gains;
+  const double* corr_rx;  // interleaved complex, may be null
+  const double* corr_tx;
+  const double* noise_power;
+  int32_t* errors;
+  float2* equalized;
+  uint64_t seed;
+  int64_t first_drop;
+  // LDS byte offsets
+  int off_x, off_coef, off_abar, off_tw, off_bits, off_cons, off_s, off_red, off_tmp_sos;
+};
+
+// bps bits of symbol q from an LSB-first word window (bit i of the result = i-th bit)
+__device__ __forceinline__ uint32_t symbol_bits(const uint32_t* bw, int q, int bps) {
+  const int bit = q * bps;
+  uint64_t v = bw[bit >> 5];
+  if ((bit & 31) + bps > 32) v |= static_cast<uint64_t>(bw[(bit >> 5) + 1]) << 32;
+  return static_cast<uint32_t>(v >> (bit & 31)) & ((1u << bps) - 1u);
+}
+
+template <int NT, int NR, int PAIRS>
+__global__ void __launch_bounds__(kBatchMaxThreads) batch_kernel(BatchArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  float2* xbuf = reinterpret_cast<float2*>(lds + a.off_x);
+  float2* coef = reinterpret_cast<float2*>(lds + a.off_coef);
+  float2* abar = reinterpret_cast<float2*>(lds + a.off_abar);
+  float2* tw = reinterpret_cast<float2*>(lds + a.off_tw);
+  uint32_t* bits = reinterpret_cast<uint32_t*>(lds + a.off_bits);
+  float2* cons = reinterpret_cast<float2*>(lds + a.off_cons);
+  float2* smat = reinterpret_cast<float2*>(lds + a.off_s);
+  int* red = reinterpret_cast<int*>(lds + a.off_red);
+  double* uni = reinterpret_cast<double*>(lds + a.off_x);        // prologue only
+  double* sos = reinterpret_cast<double*>(lds + a.off_tmp_sos);  // prologue only
+
+  const int tid = threadIdx.x;
+  const int nthreads = blockDim.x;
+  const int64_t local_drop = blockIdx.x;
+  const uint32_t drop = static_cast<uint32_t>(a.first_drop + local_drop);
+  const int n = a.n;
+  const int L = a.paths;
+
+  // ---------------------------------------------------------------- prologue
+  // realisation uniforms Phi(z), z ~ N(0,1) (consistent.py:432,485)
+  for (int c = tid; 2 * c < a.nnorm; c += nthreads) {
+    const double2 z = normal_pair_f64(stream_call(a.seed, drop, HPX_STREAM_REALIZATION, static_cast<uint32_t>(c)));
+    uni[2 * c] = normal_cdf(z.x);
+    if (2 * c + 1 < a.nnorm) uni[2 * c + 1] = normal_cdf(z.y);
+  }
+  if (tid == 0) red[0] = 0;
+  __syncthreads();
+
+  // spatial response S = R_rx exp(2 pi j U)[:NR, :NT] R_tx (fading.py:476-489)
+  for (int e = tid; e < NR * NT; e += nthreads) {
+    const int r = e / NT, t = e - r * NT;
+    double sr = 0.0, si = 0.0;
+    if (a.corr_rx) {
+      for (int p = 0; p < NR; ++p)
+        for (int q = 0; q < NT; ++q) {
+          double s0r, s0i;
+          sincospi(2.0 * uni[10 * p + q], &s0i, &s0r);
+          const double ar = a.corr_rx[2 * (r * NR + p)], ai = a.corr_rx[2 * (r * NR + p) + 1];
+          const double br = a.corr_tx[2 * (q * NT + t)], bi = a.corr_tx[2 * (q * NT + t) + 1];
+          const double mr = ar * s0r - ai * s0i, mi = ar * s0i + ai * s0r;
+          sr += mr * br - mi * bi;
+          si += mr * bi + mi * br;
+        }
+    } else {
+      sincospi(2.0 * uni[10 * r + t], &si, &sr);
+    }
+    smat[e] = make_float2(static_cast<float>(sr), static_cast<float>(si));
+  }
+  // SoS table (fading.py:326-342, angles/phases :492-499): NLOS terms, then LOS
+  {
+    const int ns = a.sinusoids;
+    const int o_los_ang = 100, o_nlos_ang = 100 + L, o_los_ph = 100 + L + L * ns, o_nlos_ph = 100 + 2 * L + L * ns;
+    const double inv_fs = 1.0 / a.bandwidth;
+    for (int e = tid; e < L * a.terms; e += nthreads) {
+      const int p = e / a.terms, k = e - p * a.terms;
+      double* row = sos + 3 * e;
+      if (k < ns) {
+        const double ang = -M_PI + 2.0 * M_PI * uni[o_nlos_ang + p * ns + k];
+        row[0] = a.doppler * cos((2.0 * M_PI * (k + 1) + ang) / ns) * inv_fs;
+        row[1] = -M_PI + 2.0 * M_PI * uni[o_nlos_ph + p * ns + k];
+        row[2] = a.path_gains[L + p];
+      } else {
+        const double ang = 2.0 * M_PI * uni[o_los_ang + p];
+        row[0] = a.los_doppler * cos(ang) * inv_fs;
+        row[1] = -M_PI + 2.0 * M_PI * uni[o_los_ph + p];
+        row[2] = a.path_gains[p];
+      }
+    }
+  }
+  // constellation indexed by transmitted bit order (scaled by sqrt(P_tx / N)); twiddles
+  for (int v = tid; v < a.order; v += nthreads) {
+    const double2 p = constellation_point(a.order, 1, bits_to_label(static_cast<uint32_t>(v), a.bps));
+    cons[v] = make_float2(static_cast<float>(p.x * a.tx_scale), static_cast<float>(p.y * a.tx_scale));
+  }
+  for (int k = tid; k < n; k += nthreads) {
+    double s, c;
+    sincospi(-2.0 * static_cast<double>(k) / static_cast<double>(n), &s, &c);
+    tw[k] = make_float2(static_cast<float>(c), static_cast<float>(s));
+  }
+  __syncthreads();
+
+  // Taylor coefficients of every (path, block) and word means of every (path, word)
+  const TaylorBlocks tb{a.log2b};
+  for (int e = tid; e < L * a.nb; e += nthreads) {
+    const int p = e / a.nb, b = e - p * a.nb;
+    taylor_coeffs(sos + 3 * p * a.terms, a.terms, tb.centre(b), tb.half_width(), coef + e * kTaylorCoeffs);
+  }
+  for (int e = tid; e < L * a.words; e += nthreads) {
+    const int p = e / a.words, w = e - p * a.words;
+    const double2 m = tap_mean(sos + 3 * p * a.terms, a.terms, static_cast<double>(w) * (n + a.cp) + a.cp, n);
+    abar[e] = make_float2(static_cast<float>(m.x), static_cast<float>(m.y));
+  }
+  __syncthreads();
+  // the signal buffer aliased the prologue tables; the frame starts after silence
+  for (int e = tid; e < NT * a.dx; e += nthreads) xbuf[(e / a.dx) * a.xs + (e % a.dx)] = make_float2(0.f, 0.f);
+
+  const double np = a.noise_power[local_drop];
+  const float sigma = static_cast<float>(sqrt(0.5 * np));
+  const float s2 = a.detector == HPX_DETECT_MMSE ? static_cast<float>(np) : 0.0f;
+  const float rx_scale = static_cast<float>(1.0 / sqrt(static_cast<double>(n)));
+  const int period = n + a.cp;
+  const int x0 = a.dx + a.cp;  // LDS column of the first data sample
+  int errors = 0;
+
+  // ---------------------------------------------------------------- words
+  for (int w = 0; w < a.words; ++w) {
+    // bits of this word for every stream (stream HPX_STREAM_BITS + t)
+    {
+      const int nw = a.bits_words;
+      const int wb0 = w * nw;
+      const int c0 = wb0 >> 2, c1 = (wb0 + nw - 1) >> 2;
+      const int ncalls = c1 - c0 + 1;
+      for (int e = tid; e < NT * ncalls; e += nthreads) {
+        const int t = e / ncalls, c = c0 + (e - t * ncalls);
+        const uint4 r = stream_call(a.seed, drop, HPX_STREAM_BITS + t, static_cast<uint32_t>(c));
+        const uint32_t comp[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int wi = 4 * c + k - wb0;
+          if (wi >= 0 && wi < nw) bits[t * nw + wi] = comp[k];
+        }
+      }
+    }
+    __syncthreads();
+    // map + place at the IFFT bins
+    for (int e = tid; e < NT * n; e += nthreads) {
+      const int t = e / n, q = e - t * n;
+      xbuf[t * a.xs + x0 + subcarrier_bin(q, n, a.dc)] = cons[symbol_bits(bits + t * a.bits_words, q, a.bps)];
+    }
+    __syncthreads();
+    lds_fft<+1>(xbuf + x0, NT, a.xs, a.log2n, tw);
+    // cyclic prefix
+    for (int e = tid; e < NT * a.cp; e += nthreads) {
+      const int t = e / a.cp, i = e - t * a.cp;
+      xbuf[t * a.xs + a.dx + i] = xbuf[t * a.xs + a.dx + n + i];
+    }
+    __syncthreads();
+
+    // channel over the data window + AWGN; y stays in registers
+    const int64_t win0 = static_cast<int64_t>(w) * period + a.cp;  // absolute index of the first data sample
+    float2 y[PAIRS][2][NR];
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) {
+      const int i = tid + k * nthreads;  // sample pair of the data window
+      if (2 * i < n) {
+#pragma unroll
+        for (int h2 = 0; h2 < 2; ++h2) {
+          const int nl = 2 * i + h2;
+          const int64_t nabs = win0 + nl;
+          float2 z[NT];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) z[t] = make_float2(0.f, 0.f);
+          for (int p = 0; p < L; ++p) {
+            const int d = a.delays[p];
+            const int64_t m = nabs - d;
+            if (d < 0 || d > a.maxd || m < 0) continue;
+            const float2 h =
+                taylor_eval(coef + (p * a.nb + static_cast<int>(tb.block_of(m))) * kTaylorCoeffs, tb.offset(m));
+            const float2* xc = xbuf + x0 + nl - d;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) z[t] = cfma(h, xc[t * a.xs], z[t]);
+          }
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc = cfma(smat[r * NT + t], z[t], acc);
+            y[k][h2][r] = acc;
+          }
+        }
+        // noise sample n uses words (0,1) (n even) or (2,3) (n odd) of call n/2
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const int64_t nabs = win0 + 2 * i;
+          if ((nabs & 1) == 0) {
+            const uint4 wn = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>(nabs >> 1));
+            const float2 n0 = complex_normal_f32(wn.x, wn.y), n1 = complex_normal_f32(wn.z, wn.w);
+            y[k][0][r] = make_float2(fmaf(sigma, n0.x, y[k][0][r].x), fmaf(sigma, n0.y, y[k][0][r].y));
+            y[k][1][r] = make_float2(fmaf(sigma, n1.x, y[k][1][r].x), fmaf(sigma, n1.y, y[k][1][r].y));
+          } else {
+            const uint4 wa = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>(nabs >> 1));
+            const uint4 wb = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>((nabs + 1) >> 1));
+            const float2 n0 = complex_normal_f32(wa.z, wa.w), n1 = complex_normal_f32(wb.x, wb.y);
+            y[k][0][r] = make_float2(fmaf(sigma, n0.x, y[k][0][r].x), fmaf(sigma, n0.y, y[k][0][r].y));
+            y[k][1][r] = make_float2(fmaf(sigma, n1.x, y[k][1][r].x), fmaf(sigma, n1.y, y[k][1][r].y));
+          }
+        }
+      }
+    }
+    // last dx transmitted samples of this word feed the next word's delayed paths
+    float2 tail[kTailPerThread];
+#pragma unroll
+    for (int k = 0; k < kTailPerThread; ++k) {
+      const int e = tid + k * nthreads;
+      if (e < NT * a.dx) tail[k] = xbuf[(e / a.dx) * a.xs + period + (e % a.dx)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kTailPerThread; ++k) {
+      const int e = tid + k * nthreads;
+      if (e < NT * a.dx) xbuf[(e / a.dx) * a.xs + (e % a.dx)] = tail[k];
+    }
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) {
+      const int i = tid + k * nthreads;
+      if (2 * i < n) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          xbuf[r * a.xs + x0 + 2 * i] = y[k][0][r];
+          xbuf[r * a.xs + x0 + 2 * i + 1] = y[k][1][r];
+        }
+      }
+    }
+    __syncthreads();
+    lds_fft<-1>(xbuf + x0, NR, a.xs, a.log2n, tw);
+
+    // ideal CSI, detection, demapping and bit errors per subcarrier
+    for (int q = tid; q < n; q += nthreads) {
+      const int bin = subcarrier_bin(q, n, a.dc);
+      float2 c = make_float2(0.f, 0.f);
+      for (int p = 0; p < L; ++p) {
+        const int d = a.delays[p];
+        if (d < 0 || d >= a.keep_below) continue;
+        c = cfma(abar[p * a.words + w], tw[(bin * d) & (n - 1)], c);
+      }
+      float2 xh[NT];
+      const float2* ycol = xbuf + x0 + bin;
+      if (NR == 1) {
+        const float2 yv = cscale(ycol[0], rx_scale);
+        float2 hs = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) hs = cadd(hs, cmul(smat[t], c));
+        const float inv = 1.0f / (cabs2(hs) + s2);
+        xh[0] = cscale(cmulc(hs, yv), inv);
+      } else {
+        detect_rows<NT>(
+            NR, [&](int r, int t) { return cmul(smat[r * NT + t], c); },
+            [&](int r) { return cscale(ycol[r * a.xs], rx_scale); }, s2, xh);
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const uint32_t rx = demap_symbol(a.mod, static_cast<double>(xh[t].x), static_cast<double>(xh[t].y));
+        const uint32_t tx = symbol_bits(bits + t * a.bits_words, q, a.bps);
+        errors += __popc(tx ^ rx);
+        if (a.equalized)
+          a.equalized[(local_drop * NT + t) * static_cast<int64_t>(a.words) * n + static_cast<int64_t>(w) * n + q] =
+              xh[t];
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---------------------------------------------------------------- epilogue
+  for (int off = 32; off > 0; off >>= 1) errors += __shfl_down(errors, off, 64);
+  if ((tid & 63) == 0) atomicAdd(red, errors);
+  __syncthreads();
+  if (tid == 0) a.errors[local_drop] = red[0];
+}
+
+int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
+
+struct Plan {
+  int threads = 0, pairs = 0;
+  int64_t lds = 0;
+  BatchArgs args{};
+};
+
+// Validates the link and computes the launch geometry and LDS layout.
+int make_plan(const hpx_link* l, Plan* out) {
+  HPX_REQUIRE(l, "hpx_run_batch: null link");
+  const int log2n = ilog2_exact(l->num_subcarriers);
+  if (log2n < 5 || l->num_subcarriers > 4096) {
+    set_error("hpx_run_batch: num_subcarriers must be a power of two in [32, 4096] (got %d)", l->num_subcarriers);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  HPX_REQUIRE(l->words >= 1 && l->prefix >= 0 && l->prefix <= l->num_subcarriers, "hpx_run_batch: invalid frame");
+  HPX_REQUIRE(l->num_paths >= 1 && l->num_sinusoids >= 1 && l->max_delay >= 0,
+              "hpx_run_batch: invalid path configuration");
+  HPX_REQUIRE(l->bandwidth > 0.0, "hpx_run_batch: bandwidth must be positive");
+  HPX_REQUIRE(l->device_power >= 0.0, "hpx_run_batch: device power must be non-negative");
+  HPX_REQUIRE(l->detector == HPX_DETECT_ZF || l->detector == HPX_DETECT_MMSE, "hpx_run_batch: unknown detector");
+  const bool known = l->num_tx == 1 || l->num_tx == 2 || l->num_tx == 4 || l->num_tx == 8;
+  if (l->num_rx != l->num_tx || !known) {
+    set_error("hpx_run_batch: antenna configuration %dx%d not instantiated (1x1, 2x2, 4x4, 8x8)", l->num_rx,
+              l->num_tx);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  Modulation mod;
+  if (l->modulation_order == 2 || !modulation(l->modulation_order, 1, &mod)) {
+    set_error("hpx_run_batch: modulation order %d not supported (4, 16, 64, 256)", l->modulation_order);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  BatchArgs& a = out->args;
+  a.n = l->num_subcarriers;
+  a.log2n = log2n;
+  a.cp = l->prefix;
+  a.words = l->words;
+  a.bps = mod.bps;
+  a.order = l->modulation_order;
+  a.mod = mod;
+  a.detector = l->detector;
+  a.dc = l->dc_suppression ? 1 : 0;
+  a.paths = l->num_paths;
+  a.sinusoids = l->num_sinusoids;
+  a.terms = l->num_sinusoids + 1;
+  a.nnorm = 100 + 2 * l->num_paths + 2 * l->num_paths * l->num_sinusoids;
+  a.frame_len = static_cast<int64_t>(l->words) * (l->num_subcarriers + l->prefix);
+  a.bandwidth = l->bandwidth;
+  a.doppler = l->doppler;
+  a.los_doppler = l->los_doppler;
+  a.tx_scale = sqrt(l->device_power) / sqrt(static_cast<double>(l->num_subcarriers));
+  const double max_rate = std::max(fabs(l->doppler), fabs(l->los_doppler)) / l->bandwidth;
+  a.log2b = taylor_log2_block(max_rate, a.terms, 11);
+  if (a.log2b < 6) {
+    set_error("hpx_run_batch: Doppler rate %.3g rad/sample too high for the block Taylor taps", max_rate);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  a.nb = static_cast<int>((a.frame_len + (int64_t(1) << a.log2b) - 1) >> a.log2b);
+  a.bits_words = (l->num_subcarriers * mod.bps) / 32;
+  a.maxd = l->max_delay;
+  a.dx = std::max(0, l->max_delay - a.cp);
+  if (a.dx > a.n) {
+    set_error("hpx_run_batch: delay spread of %d samples exceeds one OFDM word", l->max_delay);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  a.xs = a.dx + a.cp + a.n;
+  a.xs += a.xs & 1;
+  a.keep_below = static_cast<int>(std::min<int64_t>(std::min<int64_t>(1 + l->max_delay, a.frame_len), a.n));
+
+  const int nt = l->num_tx, nr = l->num_rx;
+  const int threads = std::min(kBatchMaxThreads, std::max(64, l->num_subcarriers / 2));
+  const int pairs = std::max(1, l->num_subcarriers / (2 * threads));
+  if (l->num_subcarriers * std::max(nt, nr) > kFftMaxPerThread * threads) {
+    set_error("hpx_run_batch: %d streams of %d subcarriers exceed one workgroup's FFT capacity", std::max(nt, nr),
+              l->num_subcarriers);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  if (nt * a.dx > kTailPerThread * threads) {
+    set_error("hpx_run_batch: inter-word delay tail too long (%d samples)", a.dx);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  out->threads = threads;
+  out->pairs = pairs;
+
+  // LDS layout; the prologue tables (uniforms, SoS table) alias the signal buffer
+  const int rows = std::max(nr, nt);
+  int64_t o = 0;
+  a.off_x = 0;
+  const int64_t xbytes = static_cast<int64_t>(rows) * a.xs * sizeof(float2);
+  const int64_t uni_bytes = align16(static_cast<int64_t>(a.nnorm) * sizeof(double));
+  const int64_t sos_bytes = static_cast<int64_t>(a.paths) * a.terms * 3 * sizeof(double);
+  o += align16(std::max(xbytes, uni_bytes + sos_bytes));
+  a.off_tmp_sos = static_cast<int>(uni_bytes);
+  a.off_coef = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs * sizeof(float2));
+  a.off_abar = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.paths) * a.words * sizeof(float2));
+  a.off_tw = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.n) * sizeof(float2));
+  a.off_bits = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(nt) * (a.bits_words + 1) * sizeof(uint32_t));
+  a.off_cons = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.order) * sizeof(float2));
+  a.off_s = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(nr) * nt * sizeof(float2));
+  a.off_red = static_cast<int>(o);
+  o += 16;
+  out->lds = o;
+  if (o > 160 * 1024) {
+    set_error("hpx_run_batch: link needs %lld bytes of LDS per drop (max 163840)", static_cast<long long>(o));
+    return HPX_ERR_UNSUPPORTED;
+  }
+  return HPX_OK;
+}
+
+template <class K>
+int launch(K kernel, const Plan& plan, int64_t num_drops, hipStream_t stream) {
+  if (plan.lds > 64 * 1024 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          static_cast<int>(plan.lds)) != hipSuccess) {
+    set_error("hpx_run_batch: %lld bytes of LDS per workgroup not available", static_cast<long long>(plan.lds));
+    return HPX_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(num_drops)), dim3(plan.threads), plan.lds, stream, plan.args);
+  return check_launch("hpx_run_batch");
+}
+
+template <int NT>
+int launch_pairs(const Plan& plan, int64_t num_drops, hipStream_t stream) {
+  switch (plan.pairs) {
+    case 1: return launch(batch_kernel<NT, NT, 1>, plan, num_drops, stream);
+    case 2: return launch(batch_kernel<NT, NT, 2>, plan, num_drops, stream);
+    case 4: return launch(batch_kernel<NT, NT, 4>, plan, num_drops, stream);
+  }
+  set_error("hpx_run_batch: unsupported samples per thread");
+  return HPX_ERR_UNSUPPORTED;
+}
+
+}  // namespace
+
+extern "C" int64_t hpx_run_batch_lds_bytes(const hpx_link* link_host) {
+  Plan plan;
+  if (make_plan(link_host, &plan) != HPX_OK) return 0;
+  return plan.lds;
+}
+
+extern "C" int hpx_run_batch(const hpx_link* link_host, const int32_t* delays, const double* path_gains,
+                             const hpx_c128* corr_rx, const hpx_c128* corr_tx, uint64_t seed, int64_t first_drop,
+                             int64_t num_drops, const double* noise_power, int32_t* errors, hpx_c64* equalized,
+                             void* stream) {
+  HPX_REQUIRE(delays && path_gains && noise_power && errors, "hpx_run_batch: null pointer");
+  HPX_REQUIRE((corr_rx == nullptr) == (corr_tx == nullptr), "hpx_run_batch: give both correlation matrices or none");
+  HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff, "hpx_run_batch: invalid drop count");
+  HPX_REQUIRE(first_drop >= 0 && first_drop + num_drops <= (int64_t(1) << 32),
+              "hpx_run_batch: drop index exceeds 32 bits");
+  Plan plan;
+  const int st = make_plan(link_host, &plan);
+  if (st != HPX_OK) return st;
+  if (num_drops == 0) return HPX_OK;
+  BatchArgs& a = plan.args;
+  a.delays = delays;
+  a.path_gains = path_gains;
+  a.corr_rx = reinterpret_cast<const double*>(corr_rx);
+  a.corr_tx = reinterpret_cast<const double*>(corr_tx);
+  a.noise_power = noise_power;
+  a.errors = errors;
+  a.equalized = reinterpret_cast<float2*>(equalized);
+  a.seed = seed;
+  a.first_drop = first_drop;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (link_host->num_tx) {
+    case 1: return launch_pairs<1>(plan, num_drops, s);
+    case 2: return launch_pairs<2>(plan, num_drops, s);
+    case 4: return launch_pairs<4>(plan, num_drops, s);
+    case 8: return launch_pairs<8>(plan, num_drops, s);
+  }
+  set_error("hpx_run_batch: unreachable antenna configuration");
+  return HPX_ERR_UNSUPPORTED;
+}

@@ -1,0 +1,119 @@
+"""Batched Monte Carlo drops on the GPU: the ``Simulation.run()`` drop loop of
+the reference (``hermespy/simulation/simulation.py:408-457`` driving
+``hermespy/core/pymonte/actors.py:340-452``) as fused launches of
+``hpx_run_batch``.
+
+Every drop is independent (the reference realises a fresh channel per drop,
+``simulation.py:69-70``), so a launch simulates ``drops_per_launch`` drops in
+parallel, one workgroup each, and returns their bit error counts.  The
+campaign BER of an SNR point is the mean of the per-drop BERs, as
+``BitErrorEvaluation.artifact`` / ``ScalarEvaluationResult`` aggregate it
+(``hermespy/modem/evaluators.py:231-233``, ``hermespy/core/pymonte/scalar.py:100-125``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .configs import LinkConfig
+
+
+@dataclass
+class BatchResult:
+    snr_db: np.ndarray  # per SNR point
+    errors: np.ndarray  # per SNR point, summed bit errors
+    drops: np.ndarray  # per SNR point
+    bits_per_drop: int
+
+    @property
+    def ber(self) -> np.ndarray:
+        return self.errors / np.maximum(self.drops * self.bits_per_drop, 1)
+
+
+class BatchedLink:
+    """Device-resident parameters of one link for ``hpx_run_batch``."""
+
+    def __init__(self, link: LinkConfig, device: str | torch.device = "cuda") -> None:
+        self.link = link
+        self.device = torch.device(device)
+        lib = _lib.load()
+        det = {"zf": _lib.HPX_DETECT_ZF, "mmse": _lib.HPX_DETECT_MMSE}.get(link.detector)
+        if det is None:
+            raise ValueError(f"unknown detector {link.detector}")
+        self.c_link = _lib.HpxLink(
+            link.num_rx, link.num_tx, link.num_subcarriers, link.words, link.prefix, link.modulation_order,
+            link.num_paths, link.num_sinusoids, link.max_delay, det, int(bool(link.dc_suppression)),
+            float(link.bandwidth), float(link.doppler),
+            float(link.doppler if link.los_doppler is None else link.los_doppler), float(link.gain),
+            float(link.device_power))
+        self.lds_bytes = int(lib.hpx_run_batch_lds_bytes(ctypes.byref(self.c_link)))
+        if self.lds_bytes <= 0:
+            # surface the validation message
+            _lib.check(_lib.HPX_ERR_UNSUPPORTED)
+        dev = self.device
+        self.delays = torch.as_tensor(link.delay_samples, dtype=torch.int32, device=dev)
+        self.path_gains = torch.as_tensor(link.path_gains, dtype=torch.float64, device=dev).contiguous()
+        if link.corr_rx is not None:
+            rrx = np.asarray(link.corr_rx, dtype=np.complex128)[: link.num_rx, : link.num_rx]
+            rtx = np.asarray(link.corr_tx, dtype=np.complex128)[: link.num_tx, : link.num_tx]
+            self.corr_rx = torch.as_tensor(np.ascontiguousarray(rrx), device=dev)
+            self.corr_tx = torch.as_tensor(np.ascontiguousarray(rtx), device=dev)
+        else:
+            self.corr_rx = self.corr_tx = None
+
+    def run(self, seed: int, first_drop: int, noise_power: torch.Tensor, errors: torch.Tensor,
+            equalized: torch.Tensor | None = None, stream: int | None = None) -> None:
+        """Launch ``noise_power.numel()`` drops (asynchronous on the current stream)."""
+
+        n = int(noise_power.numel())
+        if errors.numel() < n or errors.dtype != torch.int32:
+            raise ValueError("errors must be an int32 tensor with one entry per drop")
+        if noise_power.dtype != torch.float64:
+            raise ValueError("noise_power must be float64")
+        if equalized is not None:
+            need = n * self.link.num_tx * self.link.words * self.link.num_subcarriers
+            if equalized.dtype != torch.complex64 or equalized.numel() < need:
+                raise ValueError("equalized must be complex64 with num_drops * num_tx * words * N entries")
+        lib = _lib.load()
+        st = lib.hpx_run_batch(ctypes.byref(self.c_link), _lib.ptr(self.delays), _lib.ptr(self.path_gains),
+                               _lib.ptr(self.corr_rx), _lib.ptr(self.corr_tx), ctypes.c_uint64(seed & (2**64 - 1)),
+                               int(first_drop), n, _lib.ptr(noise_power), _lib.ptr(errors), _lib.ptr(equalized),
+                               _lib.stream_handle() if stream is None else stream)
+        _lib.check(st)
+
+
+class BatchedSimulation:
+    """Monte Carlo BER campaign over an SNR grid, ``drops_per_launch`` drops per fused launch.
+
+    Drop ``i`` of the campaign uses SNR point ``i % len(snr_db)`` and Philox
+    counter ``i``, so any drop can be re-simulated on its own.
+    """
+
+    def __init__(self, link: LinkConfig, seed: int = 20261202, device: str = "cuda") -> None:
+        self.link = link
+        self.seed = int(seed)
+        self.batched = BatchedLink(link, device)
+        self.device = self.batched.device
+
+    def noise_powers(self, first_drop: int, count: int, snr_db) -> torch.Tensor:
+        snr = np.asarray(snr_db, dtype=np.float64)
+        idx = (first_drop + np.arange(count)) % snr.size
+        return torch.as_tensor(self.link.noise_power(snr[idx]), dtype=torch.float64, device=self.device)
+
+    def run(self, num_drops: int, snr_db=None, drops_per_launch: int = 4096, first_drop: int = 0) -> BatchResult:
+        snr = np.asarray(self.link.snr_db if snr_db is None else snr_db, dtype=np.float64)
+        errors = torch.empty(num_drops, dtype=torch.int32, device=self.device)
+        for start in range(0, num_drops, drops_per_launch):
+            cnt = min(drops_per_launch, num_drops - start)
+            npw = self.noise_powers(first_drop + start, cnt, snr)
+            self.batched.run(self.seed, first_drop + start, npw, errors[start:start + cnt])
+        err = errors.cpu().numpy().astype(np.int64)
+        idx = (first_drop + np.arange(num_drops)) % snr.size
+        per_err = np.bincount(idx, weights=err, minlength=snr.size).astype(np.int64)
+        per_drops = np.bincount(idx, minlength=snr.size).astype(np.int64)
+        return BatchResult(snr, per_err, per_drops, self.link.bits_per_drop)

@@ -1,0 +1,78 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, and its host-side mapping/demapping twins are bit-identical to
+the oracle's restatement of the reference."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import mapping
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "hermespy_amd.h"
+MODS = [(2, 1), (4, 1), (16, 1), (64, 1), (256, 1), (4, 0), (8, 0), (16, 0)]
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(hpx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for required in ["hpx_fading_taps", "hpx_tdl_propagate", "hpx_awgn", "hpx_map_symbols", "hpx_demap_hard",
+                     "hpx_ofdm_modulate", "hpx_ofdm_demodulate", "hpx_ideal_csi", "hpx_equalize",
+                     "hpx_count_errors", "hpx_run_batch"]:
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol(native_lib):
+    from hermespy_amd import _lib
+
+    for name in declared_functions():
+        assert hasattr(native_lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+    assert native_lib.hpx_abi_version() == 1
+
+
+def test_error_channel(native_lib):
+    from hermespy_amd import _lib
+
+    with pytest.raises(ValueError, match="not supported"):
+        _lib.check(native_lib.hpx_constellation_host(3, 1, ctypes.c_void_p(0)) if False else
+                   native_lib.hpx_constellation_host(12, 1, np.zeros(24).ctypes.data))
+    with pytest.raises(ValueError, match="null pointer"):
+        _lib.check(native_lib.hpx_fading_taps(None, 1, 1, 10, None, None))
+
+
+@pytest.mark.parametrize("order,is_complex", MODS)
+def test_host_constellation_bit_exact(native_lib, order, is_complex):
+    pts = np.zeros(2 * order, dtype=np.float64)
+    assert native_lib.hpx_constellation_host(order, is_complex, pts.ctypes.data) == 0
+    got = pts[0::2] + 1j * pts[1::2]
+    exp = mapping.constellation(order, bool(is_complex))
+    assert np.array_equal(got.view(np.float64), exp.astype(np.complex128).view(np.float64))
+
+
+@pytest.mark.parametrize("order,is_complex", MODS)
+def test_host_demap_bit_exact(native_lib, order, is_complex):
+    """Random points, every decision threshold and its float64 neighbours."""
+
+    rng = np.random.default_rng(order + 7 * is_complex)
+    pts = mapping.constellation(order, bool(is_complex))
+    lim = 1.5 * np.max(np.abs(pts.real))
+    x = rng.uniform(-lim, lim, 4000) + 1j * rng.uniform(-lim, lim, 4000)
+    # thresholds in the scaled domain r = x * sqrt(2) * sqrt(M') are integers
+    pam = {2: 2, 4: 2 if is_complex else 4, 16: 4 if is_complex else 16, 64: 8, 256: 16, 8: 8}[order]
+    scale = (np.sqrt(2) if (is_complex and order > 2) else 1.0) * {2: 1.0, 4: np.sqrt(5), 8: np.sqrt(21),
+                                                                   16: np.sqrt(85)}[pam]
+    thr = np.arange(-pam, pam + 1, dtype=np.float64) / scale
+    edge = np.concatenate([thr, np.nextafter(thr, np.inf), np.nextafter(thr, -np.inf)])
+    x = np.concatenate([x, edge + 1j * edge[::-1], edge + 0.3j, 0.3 + 1j * edge])
+    bits = np.zeros(x.size * int(np.log2(order)), dtype=np.uint8)
+    xx = np.ascontiguousarray(np.stack([x.real, x.imag], -1))
+    assert native_lib.hpx_demap_hard_host(xx.ctypes.data, x.size, order, is_complex, bits.ctypes.data) == 0
+    exp = mapping.detect_bits(x if is_complex else x.real + 0j, order, bool(is_complex)).astype(np.uint8)
+    assert np.array_equal(bits, exp)

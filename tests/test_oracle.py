@@ -1,0 +1,292 @@
+"""Pins the CPU oracle against golden vectors, closed forms and library routines.
+
+Each check is chosen so that a plausible restatement mistake (dropped term,
+wrong sign or index, transposed operand) fails it.
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy.fft import fft, ifft
+
+from conftest import rel_err
+from oracle import csi, drop, equalize, fading, mapping, ofdm, philox
+
+GOLDEN = Path(__file__).parent / "golden"
+
+
+# ------------------------------------------------------------------ philox
+@pytest.mark.parametrize("ctr,key,expected", [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+])
+def test_philox_known_answers(ctr, key, expected):
+    """Random123 known-answer vectors of Philox4x32-10."""
+
+    out = philox.philox4x32(np.array(ctr, dtype=np.uint32), np.array(key, dtype=np.uint32))
+    assert out.tolist() == list(expected)
+
+
+def test_philox_streams_statistics():
+    z = philox.realization_normals(1, 7, 200001)
+    assert abs(z.mean()) < 0.01 and abs(z.var() - 1) < 0.01
+    n = philox.noise_normals(1, 7, 0, 100001)
+    assert abs(np.var(n.real) - 1) < 0.02 and abs(np.var(n.imag) - 1) < 0.02
+    b = philox.stream_bits(1, 7, 0, 200000)
+    assert abs(b.mean() - 0.5) < 0.005
+    # distinct streams and drops differ
+    assert not np.array_equal(philox.stream_bits(1, 7, 1, 64), philox.stream_bits(1, 7, 0, 64))
+    assert not np.array_equal(philox.stream_bits(1, 8, 0, 64), philox.stream_bits(1, 7, 0, 64))
+
+
+# ------------------------------------------------------------------ mapping
+KATS = json.loads((GOLDEN / "psk_qam_kat.json").read_text())
+
+
+@pytest.mark.parametrize("case", KATS, ids=[c["name"] for c in KATS])
+def test_mapping_reference_kat(case):
+    """tests/unit_tests/modem/tools/test_psk_qam_mapping.py:80-144 of the reference."""
+
+    sym = mapping.get_symbols(np.array(case["bits"], dtype=np.uint8), case["order"], case["is_complex"])
+    exp = np.array([complex(r, i) for r, i in case["symbols"]]) / np.sqrt(case["norm"])
+    np.testing.assert_allclose(sym, exp, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("order,is_complex", [(2, True), (4, True), (16, True), (64, True), (256, True),
+                                              (4, False), (8, False), (16, False)])
+def test_constellation_unit_energy_and_gray(order, is_complex):
+    pts = mapping.constellation(order, is_complex)
+    assert abs(np.mean(np.abs(pts) ** 2) - 1) < 1e-12
+    # Gray: nearest neighbours differ in exactly one bit
+    bps = int(np.log2(order))
+    d = np.abs(pts[:, None] - pts[None, :])
+    np.fill_diagonal(d, np.inf)
+    dmin = d.min()
+    for a, b in zip(*np.nonzero(np.abs(d - dmin) < 1e-9)):
+        assert bin(a ^ b).count("1") == 1, (a, b)
+    assert len(set(np.round(pts, 12))) == order and bps >= 1
+
+
+@pytest.mark.parametrize("order,is_complex", [(2, True), (4, True), (16, True), (64, True), (256, True),
+                                              (4, False), (8, False), (16, False)])
+def test_demap_equals_nearest_point(order, is_complex):
+    """Hard LLR decisions equal maximum-likelihood (nearest point) decisions for the Gray grids."""
+
+    rng = np.random.default_rng(order)
+    pts = mapping.constellation(order, is_complex)
+    lim = 1.3 * np.max(np.abs(pts.real))
+    x = rng.uniform(-lim, lim, 20000) + (1j * rng.uniform(-lim, lim, 20000) if is_complex and order > 2 else 0)
+    bits = mapping.detect_bits(x, order, is_complex).reshape(-1, int(np.log2(order)))
+    if is_complex and order > 2:
+        near = np.argmin(np.abs(x[:, None] - pts[None, :]), axis=1)
+    else:
+        near = np.argmin(np.abs(x.real[:, None] - pts.real[None, :]), axis=1)
+    bps = int(np.log2(order))
+    labels = (bits * (1 << np.arange(bps - 1, -1, -1))).sum(axis=1)
+    assert np.array_equal(labels, near)
+
+
+def test_demap_round_trip_low_noise():
+    """Reference test_psk_qam_mapping.py:178-199: noise below half the grid spacing gives no errors."""
+
+    rng = np.random.default_rng(3)
+    for order, nv in [(4, 1 / 20), (16, 1 / 100), (64, 1 / 420), (256, 1 / 1700)]:
+        bits = rng.integers(0, 2, 100 * int(np.log2(order))).astype(np.uint8)
+        s = mapping.get_symbols(bits, order)
+        noise = (rng.standard_normal(s.shape) + 1j * rng.standard_normal(s.shape)) * np.sqrt(nv)
+        noise *= min(1.0, 0.9 / np.sqrt(2 * order) / np.abs(noise).max() * np.sqrt(2))
+        assert np.array_equal(mapping.detect_bits(s + noise, order).astype(np.uint8), bits)
+
+
+def test_demap_ties_go_to_zero():
+    """LLR == 0 at the decision boundaries -> bit 0 (``llr > 0``)."""
+
+    assert mapping.detect_bits(np.array([0.0 + 0.0j]), 4).tolist() == [False, False]
+    b = mapping.detect_bits(np.array([2 / np.sqrt(10) * (1 + 0j)]), 16)
+    # r = 2 exactly on the |r| > 2 boundary of bit 2 (up to rounding of sqrt(2) sqrt(5))
+    assert b[0] == False  # noqa: E712
+
+
+# ------------------------------------------------------------------ OFDM
+def test_ofdm_against_direct_dft():
+    """Forward/backward transformation equal an O(N^2) DFT with the bin map."""
+
+    n = 16
+    rng = np.random.default_rng(0)
+    frame = ofdm.simple_frame(n, 3, 0.25)
+    grid = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    bins = ofdm.subcarrier_bins(n, True)
+    k = np.arange(n)
+    w = np.exp(2j * np.pi * np.outer(k, k) / n)
+    spec = np.zeros((3, n), complex)
+    spec[:, bins] = grid / np.sqrt(n)
+    td = spec @ w.T  # x[m] = sum_b X[b] e^{+2 pi j b m / N}
+    np.testing.assert_allclose(ofdm.forward_transformation(frame, grid), td, atol=1e-12)
+    back = ofdm.backward_transformation(frame, td)
+    np.testing.assert_allclose(back, grid, atol=1e-12)
+    # bins are a permutation and the DC roll moves exactly the upper half by one
+    assert sorted(bins.tolist()) == list(range(n))
+    assert bins[0] == n // 2 and bins[n - 1] == 0 and ofdm.subcarrier_bins(n, False)[n // 2] == 0
+
+
+def test_ofdm_round_trip_and_power():
+    """Reference test_ofdm.py:54-100 / test_waveform.py:475-530 invariants."""
+
+    rng = np.random.default_rng(1)
+    frame = ofdm.simple_frame(64, 14, 0.25)
+    bits = rng.integers(0, 2, frame.num_data_symbols * 2).astype(np.uint8)
+    sym = mapping.get_symbols(bits, 4)
+    x = ofdm.modulate(frame, ofdm.place(frame, sym))
+    assert x.size == 14 * 80
+    # unit power per data sample (Parseval; QPSK symbols have unit modulus)
+    assert abs(np.mean(np.abs(ofdm.pick_samples(frame, x)) ** 2) - 1) < 1e-12
+    back = ofdm.pick(frame, ofdm.demodulate(frame, x))
+    np.testing.assert_allclose(back, sym, atol=1e-12)
+    # cyclic prefix repeats the word's tail
+    np.testing.assert_allclose(x[:16], x[64:80], atol=0)
+
+
+# ------------------------------------------------------------------ fading
+def _sample(L=4, N=8, K=None, doppler=2 * np.pi * 300, fs=1e5, nrx=2, ntx=2, seed=5):
+    params = fading.make_parameters(np.arange(L) / fs, np.ones(L) / L, np.zeros(L) if K is None else K, N, doppler)
+    z = np.random.default_rng(seed).standard_normal(params.num_normals)
+    return fading.sample_from_normals(params, z, nrx, ntx, fs)
+
+
+def test_taps_closed_form_single_sinusoid():
+    """One NLOS sinusoid and no LOS: h[m] = sqrt(gP) e^{j(w cos((2 pi + a)/1) m / fs + phi)}."""
+
+    s = _sample(L=1, N=1, K=np.zeros(1))
+    m = np.arange(50)
+    h = fading.taps(s, 50)[0]
+    rate = s.params.doppler * np.cos((2 * np.pi + s.nlos_angles[0, 0]) / 1) / s.bandwidth
+    los = np.sqrt(0.0) * 0  # K = 0 -> los gain 0
+    exp = np.exp(1j * (rate * m + s.nlos_phases[0, 0])) + los
+    np.testing.assert_allclose(h, exp, atol=1e-12)
+
+
+def test_taps_pure_los_and_static():
+    s = _sample(L=2, K=np.array([np.inf, np.inf]))
+    h = fading.taps(s, 100)
+    assert np.allclose(np.abs(h), np.sqrt(0.5))  # pure LOS phasor with |h|^2 = g P
+    s0 = _sample(doppler=0.0)
+    h0 = fading.taps(s0, 64)
+    assert np.allclose(h0, h0[:, :1])  # omega = 0 -> constant
+
+
+def test_taps_mean_power():
+    """E|h|^2 = g P over many realisations (Rayleigh)."""
+
+    p = []
+    for sd in range(300):
+        p.append(np.abs(fading.taps(_sample(L=1, seed=sd), 1)[0, 0]) ** 2)
+    assert abs(np.mean(p) - 1.0) < 0.2
+
+
+def test_propagate_brute_force_and_state():
+    """_propagate equals a direct double sum and equals propagation through state()
+    (reference test_fading.py:167-175)."""
+
+    s = _sample()
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((2, 40)) + 1j * rng.standard_normal((2, 40))
+    y = fading.propagate(s, x)
+    h = fading.taps(s, 40)
+    d = fading.delay_samples(s)
+    ref = np.zeros((2, 43), complex)
+    for r in range(2):
+        for n in range(43):
+            for l in range(4):
+                m = n - d[l]
+                if 0 <= m < 40:
+                    ref[r, n] += sum(s.spatial_response[r, t] * h[l, m] * x[t, m] for t in range(2))
+    np.testing.assert_allclose(y, ref, atol=1e-12)
+    st = fading.state(s, 40, 1 + 3)
+    np.testing.assert_allclose(fading.propagate_state(st, x), y, atol=1e-12)
+
+
+def test_propagate_delay_shift_and_gain():
+    """Reference test_fading.py:415-441 (delay shift) and :560-591 (gain scaling)."""
+
+    fs = 1e5
+    p1 = fading.make_parameters([0.0], [1.0], [0.0], 8, 2 * np.pi * 100)
+    p2 = fading.make_parameters([3 / fs], [1.0], [0.0], 8, 2 * np.pi * 100)
+    p3 = fading.make_parameters([0.0], [1.0], [0.0], 8, 2 * np.pi * 100, gain=10.0)
+    z = np.random.default_rng(9).standard_normal(p1.num_normals)
+    s1, s2, s3 = (fading.sample_from_normals(p, z, 1, 1, fs) for p in (p1, p2, p3))
+    x = np.exp(2j * np.pi * 0.01 * np.arange(30))[None]
+    y1, y2, y3 = fading.propagate(s1, x), fading.propagate(s2, x), fading.propagate(s3, x)
+    np.testing.assert_allclose(y2[:, 3:], y1, atol=1e-12)
+    np.testing.assert_allclose(y3, y1 * np.sqrt(10.0), atol=1e-12)
+
+
+def test_spatial_response_correlation():
+    """S = R_rx exp(2 pi j U) R_tx with U the first 100 uniforms row-major (fading.py:476-489)."""
+
+    from scipy.special import ndtr
+
+    r = np.array([[1, 0.3], [0.3, 1]], complex)
+    p = fading.make_parameters([0.0], [1.0], [0.0], 4, 1.0, corr_rx=r, corr_tx=r)
+    z = np.random.default_rng(4).standard_normal(p.num_normals)
+    s = fading.sample_from_normals(p, z, 2, 2, 1e3)
+    u = ndtr(z[:100]).reshape(10, 10)
+    np.testing.assert_allclose(s.spatial_response, r @ np.exp(2j * np.pi * u[:2, :2]) @ r, atol=1e-14)
+
+
+# ------------------------------------------------------------------ CSI
+@pytest.mark.parametrize("doppler", [0.0, 2 * np.pi * 50, 2 * np.pi * 3000])
+def test_ideal_csi_closed_form_equals_dense_average(doppler):
+    params = fading.make_parameters(np.array([0, 2, 5, 9]) / 1e6, [0.4, 0.3, 0.2, 0.1], [1.0, 0, 0, 0], 6, doppler)
+    z = np.random.default_rng(11).standard_normal(params.num_normals)
+    s = fading.sample_from_normals(params, z, 2, 2, 1e6)
+    frame = ofdm.simple_frame(32, 5, 0.125)
+    a = csi.reference_ideal_csi(s, frame)
+    b = csi.closed_form_ideal_csi(s, frame)
+    assert rel_err(b, a) < 1e-12
+
+
+def test_ideal_csi_static_channel_gives_error_free_link():
+    """omega = 0, no noise, delay spread within the CP: ZF with ideal CSI is exact (BER 0)."""
+
+    link = drop.Link(2, 2, 64, 4, 0.25, 16, 1e6, np.arange(4) / 1e6, np.ones(4) / 4, np.zeros(4), 0.0, detector="zf")
+    r = drop.run_drop(link, 1, 0, 30.0, noise=False)
+    assert r.errors == 0
+
+
+# ------------------------------------------------------------------ equalisation / BER
+def test_zf_is_pinv_and_mmse_limit():
+    rng = np.random.default_rng(6)
+    h = rng.standard_normal((4, 3, 50)) + 1j * rng.standard_normal((4, 3, 50))
+    y = rng.standard_normal((4, 50)) + 1j * rng.standard_normal((4, 50))
+    zf = equalize.zero_forcing(y, h)
+    for k in range(50):
+        np.testing.assert_allclose(zf[:, k], np.linalg.lstsq(h[:, :, k], y[:, k], rcond=None)[0], atol=1e-10)
+    assert rel_err(equalize.mmse(y, h, 1e-12), zf) < 1e-9
+    # MMSE equals the regularised least squares of the stacked system [H; sigma I]
+    s2 = 0.3
+    mm = equalize.mmse(y, h, s2)
+    for k in range(5):
+        a = np.vstack([h[:, :, k], np.sqrt(s2) * np.eye(3)])
+        b = np.concatenate([y[:, k], np.zeros(3)])
+        np.testing.assert_allclose(mm[:, k], np.linalg.lstsq(a, b, rcond=None)[0], atol=1e-10)
+
+
+def test_bit_errors_popcount():
+    rng = np.random.default_rng(8)
+    a = rng.integers(0, 2, 1000).astype(np.uint8)
+    b = rng.integers(0, 2, 1000).astype(np.uint8)
+    packed = np.bitwise_xor(np.packbits(a), np.packbits(b))
+    assert int(np.unpackbits(packed).sum()) == int(np.abs(a.astype(np.int8) - b.astype(np.int8)).sum())
+
+
+def test_drop_faithful_equals_fast():
+    link = drop.Link(2, 2, 64, 6, 0.125, 16, 1e6, np.array([0, 1, 3, 10]) / 1e6, [0.4, 0.3, 0.2, 0.1],
+                     [1.0, 0, 0, 0], 2 * np.pi * 200, corr_rx=np.eye(2), corr_tx=np.eye(2))
+    a = drop.run_drop(link, 7, 3, 15.0, faithful=True)
+    b = drop.run_drop(link, 7, 3, 15.0, faithful=False)
+    assert rel_err(b.equalized, a.equalized) < 1e-10
+    assert a.errors == b.errors
