@@ -87,11 +87,11 @@ __global__ void __launch_bounds__(256) fading_taps_kernel(const double* __restri
 
 extern "C" int hpx_fading_taps(const double* sos, int num_paths, int num_terms, int64_t num_samples, hpx_c64* taps,
                                void* stream) {
-  HPX_REQUIRE(sos && taps, "hpx_fading_taps: null pointer");
   HPX_REQUIRE(num_paths > 0 && num_paths <= 65535 && num_terms > 0 && num_samples >= 0,
               "hpx_fading_taps: invalid sizes (paths %d, terms %d, samples %lld)", num_paths, num_terms,
               (long long)num_samples);
   if (num_samples == 0) return HPX_OK;
+  HPX_REQUIRE(sos && taps, "hpx_fading_taps: null pointer");
   const dim3 grid(static_cast<unsigned>((num_samples + 255) / 256), num_paths);
   hipLaunchKernelGGL(fading_taps_kernel, grid, dim3(256), 0, HPX_STREAM(stream), sos, num_terms, num_samples,
                      reinterpret_cast<float2*>(taps));
@@ -208,12 +208,12 @@ __global__ void __launch_bounds__(256) awgn_kernel(float2* __restrict__ sig, int
 
 extern "C" int hpx_awgn(hpx_c64* signal, int num_streams, int64_t ld, int64_t num_samples, uint64_t seed,
                         uint32_t drop, double power, void* stream) {
-  HPX_REQUIRE(signal, "hpx_awgn: null pointer");
   HPX_REQUIRE(num_streams >= 1 && num_streams <= 65535 && ld >= num_samples && num_samples >= 0,
               "hpx_awgn: invalid sizes");
   HPX_REQUIRE(power >= 0.0, "hpx_awgn: Noise power of a noise realization must be non-negative.");
   HPX_REQUIRE(num_samples <= (int64_t(1) << 33), "hpx_awgn: too many samples for the 32-bit counter");
   if (num_samples == 0 || power == 0.0) return HPX_OK;
+  HPX_REQUIRE(signal, "hpx_awgn: null pointer");
   const int64_t pairs = (num_samples + 1) / 2;
   const dim3 grid(static_cast<unsigned>((pairs + 255) / 256), num_streams);
   hipLaunchKernelGGL(awgn_kernel, grid, dim3(256), 0, HPX_STREAM(stream), reinterpret_cast<float2*>(signal), ld,
@@ -270,13 +270,14 @@ __global__ void __launch_bounds__(256) map_kernel(const uint8_t* __restrict__ bi
 
 extern "C" int hpx_map_symbols(const uint8_t* bits, int64_t num_symbols, int modulation_order, int is_complex,
                                hpx_c64* symbols, void* stream) {
-  HPX_REQUIRE(bits && symbols, "hpx_map_symbols: null pointer");
   Modulation m;
   if (!modulation(modulation_order, is_complex, &m)) {
     set_error("Modulation (%d) not supported", modulation_order);
     return HPX_ERR_UNSUPPORTED;
   }
-  if (num_symbols <= 0) return HPX_OK;
+  HPX_REQUIRE(num_symbols >= 0, "hpx_map_symbols: negative length");
+  if (num_symbols == 0) return HPX_OK;
+  HPX_REQUIRE(bits && symbols, "hpx_map_symbols: null pointer");
   hipLaunchKernelGGL(map_kernel, dim3(static_cast<unsigned>((num_symbols + 255) / 256)), dim3(256), 0,
                      HPX_STREAM(stream), bits, num_symbols, modulation_order, is_complex, m.bps,
                      reinterpret_cast<float2*>(symbols));
@@ -294,13 +295,14 @@ __global__ void __launch_bounds__(256) demap_kernel(const float2* __restrict__ s
 
 extern "C" int hpx_demap_hard(const hpx_c64* symbols, int64_t num_symbols, int modulation_order, int is_complex,
                               uint8_t* bits, void* stream) {
-  HPX_REQUIRE(symbols && bits, "hpx_demap_hard: null pointer");
   Modulation m;
   if (!modulation(modulation_order, is_complex, &m)) {
     set_error("Unsupported modulation scheme");
     return HPX_ERR_UNSUPPORTED;
   }
-  if (num_symbols <= 0) return HPX_OK;
+  HPX_REQUIRE(num_symbols >= 0, "hpx_demap_hard: negative length");
+  if (num_symbols == 0) return HPX_OK;
+  HPX_REQUIRE(symbols && bits, "hpx_demap_hard: null pointer");
   hipLaunchKernelGGL(demap_kernel, dim3(static_cast<unsigned>((num_symbols + 255) / 256)), dim3(256), 0,
                      HPX_STREAM(stream), reinterpret_cast<const float2*>(symbols), num_symbols, m, bits);
   return check_launch("hpx_demap_hard");
@@ -473,11 +475,11 @@ __global__ void __launch_bounds__(256) equalize_siso_kernel(const float2* __rest
 
 extern "C" int hpx_equalize(const hpx_c64* y, const hpx_c64* h, int num_rx, int num_tx, int64_t num_re, int mode,
                             double noise_variance, hpx_c64* x, void* stream) {
-  HPX_REQUIRE(y && h && x, "hpx_equalize: null pointer");
   HPX_REQUIRE(mode == HPX_DETECT_ZF || mode == HPX_DETECT_MMSE, "hpx_equalize: unknown mode %d", mode);
   HPX_REQUIRE(num_rx >= 1 && num_tx >= 1 && num_re >= 0, "hpx_equalize: invalid sizes");
   HPX_REQUIRE(noise_variance >= 0.0, "hpx_equalize: noise variance must be non-negative");
   if (num_re == 0) return HPX_OK;
+  HPX_REQUIRE(y && h && x, "hpx_equalize: null pointer");
   const dim3 grid(static_cast<unsigned>((num_re + 255) / 256));
   const float2* yy = reinterpret_cast<const float2*>(y);
   const float2* hh = reinterpret_cast<const float2*>(h);
@@ -531,9 +533,9 @@ __global__ void __launch_bounds__(256) count_errors_kernel(const uint8_t* __rest
 
 extern "C" int hpx_count_errors(const uint8_t* a, const uint8_t* b, int64_t num_bits, unsigned long long* errors,
                                 void* stream) {
-  HPX_REQUIRE(a && b && errors, "hpx_count_errors: null pointer");
   HPX_REQUIRE(num_bits >= 0, "hpx_count_errors: negative length");
   if (num_bits == 0) return HPX_OK;
+  HPX_REQUIRE(a && b && errors, "hpx_count_errors: null pointer");
   const int64_t work = std::max<int64_t>(1, num_bits / 4);
   const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((work + 255) / 256, 4096));
   hipLaunchKernelGGL(count_errors_kernel, dim3(blocks), dim3(256), 0, HPX_STREAM(stream), a, b, num_bits, errors);

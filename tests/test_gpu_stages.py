@@ -93,7 +93,8 @@ def test_map_demap_bit_exact(cuda, order, is_complex):
     rng = np.random.default_rng(order)
     bits = rng.integers(0, 2, 4096 * bps).astype(np.uint8)
     sym = torch.empty(4096, dtype=torch.complex64, device="cuda")
-    _lib.check(lib.hpx_map_symbols(_lib.ptr(dev(bits)), 4096, order, is_complex, _lib.ptr(sym), _lib.stream_handle()))
+    tbits = dev(bits)
+    _lib.check(lib.hpx_map_symbols(_lib.ptr(tbits), 4096, order, is_complex, _lib.ptr(sym), _lib.stream_handle()))
     exp = mapping.get_symbols(bits, order, bool(is_complex)).astype(np.complex64)
     assert np.array_equal(sym.cpu().numpy().view(np.float32), exp.view(np.float32))
     # demap: noisy points + thresholds (float32 inputs; the oracle sees the same values in float64)
@@ -102,7 +103,8 @@ def test_map_demap_bit_exact(cuda, order, is_complex):
     extra = (thr[:, None] + 1j * thr[None, :]).reshape(-1).astype(np.complex64)
     pts = np.concatenate([noisy, extra, np.array([0, -0.0, np.float32(1e-40)], np.complex64)])
     out = torch.empty(pts.size * bps, dtype=torch.uint8, device="cuda")
-    _lib.check(lib.hpx_demap_hard(_lib.ptr(dev(pts)), pts.size, order, is_complex, _lib.ptr(out), _lib.stream_handle()))
+    tpts = dev(pts)
+    _lib.check(lib.hpx_demap_hard(_lib.ptr(tpts), pts.size, order, is_complex, _lib.ptr(out), _lib.stream_handle()))
     ref = mapping.detect_bits(pts.astype(np.complex128) if is_complex else pts.real.astype(np.float64) + 0j, order,
                               bool(is_complex)).astype(np.uint8)
     assert np.array_equal(out.cpu().numpy(), ref)
@@ -119,7 +121,8 @@ def test_ofdm_modulate_demodulate(cuda, n, cp, words, streams):
     grid = grid.astype(np.complex64)
     m = frame.samples_per_frame
     out = torch.empty((streams, m), dtype=torch.complex64, device="cuda")
-    _lib.check(lib.hpx_ofdm_modulate(_lib.ptr(dev(grid)), streams, words, n, cp, 1, _lib.ptr(out), m,
+    tgrid = dev(grid)
+    _lib.check(lib.hpx_ofdm_modulate(_lib.ptr(tgrid), streams, words, n, cp, 1, _lib.ptr(out), m,
                                      _lib.stream_handle()))
     exp = np.stack([ofdm.modulate(frame, grid[s].astype(np.complex128)) for s in range(streams)])
     assert rel_err(out.cpu().numpy(), exp) < 1e-5
@@ -159,7 +162,8 @@ def test_equalize(cuda, nrx, ntx, mode):
     y = ((rng.standard_normal((nrx, k)) + 1j * rng.standard_normal((nrx, k))) / np.sqrt(2)).astype(np.complex64)
     s2 = 0.05
     out = torch.empty((ntx, k), dtype=torch.complex64, device="cuda")
-    _lib.check(_lib.load().hpx_equalize(_lib.ptr(dev(y)), _lib.ptr(dev(h)), nrx, ntx, k, mode, s2, _lib.ptr(out),
+    ty, th = dev(y), dev(h)  # keep the uploads alive until the kernel has run
+    _lib.check(_lib.load().hpx_equalize(_lib.ptr(ty), _lib.ptr(th), nrx, ntx, k, mode, s2, _lib.ptr(out),
                                         _lib.stream_handle()))
     h64, y64 = h.astype(np.complex128), y.astype(np.complex128)
     exp = oeq.zero_forcing(y64, h64) if mode == 0 else oeq.mmse(y64, h64, s2)
