@@ -64,6 +64,28 @@ def cpu_baseline(cfg, budget_s: float = 15.0) -> dict:
                       f"{dt:.1f} s on 1 host core of {os.cpu_count()}"}
 
 
+def plain_gpu_path(cfg, budget_s: float = 5.0) -> dict:
+    """The plain single-drop GPU path: stage entry points, one drop at a time."""
+
+    import torch
+
+    from hermespy_amd.simulation import SingleDropLink
+
+    plain = SingleDropLink(cfg, SEED)
+    snr = np.asarray(cfg.snr_db)
+    acc = torch.zeros(1, dtype=torch.int64, device=plain.device)
+    plain.run_drop(0, float(snr[0]), acc)  # warm-up
+    torch.cuda.synchronize()
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        plain.run_drop(done, float(snr[done % snr.size]), acc)
+        done += 1
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "drops/s", "drops": done}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +94,9 @@ def main() -> int:
     ap.add_argument("--drops", type=int, default=DROPS_PER_STEP)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain single-drop GPU path")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (counter calibration for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (drops shard across GPUs by "
@@ -94,6 +119,12 @@ def main() -> int:
            for f in first]
     errors = torch.zeros((total, n), dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib * 262144, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for s in range(args.warmup):
         link.run(SEED, first[s], npw[s], errors[s])
     torch.cuda.synchronize()
@@ -138,6 +169,10 @@ def main() -> int:
                      "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": None,
                      "kernel": "batch_kernel<4,4,1>", "flops_per_drop": algorithmic_flops_per_drop(cfg)},
     }
+    if not args.no_plain:
+        plain = plain_gpu_path(cfg)
+        out["plain_gpu_path"] = plain
+        out["speedup_vs_plain_gpu_path"] = drops_per_s / plain["value"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -74,6 +74,9 @@ SIGNATURES = {
     "hpx_run_batch": (c_int, [ctypes.POINTER(HpxLink), c_vp, c_vp, c_vp, c_vp, c_u64, c_i64, c_i64, c_vp, c_vp, c_vp,
                               c_vp]),
     "hpx_run_batch_lds_bytes": (c_i64, [ctypes.POINTER(HpxLink)]),
+    "hpx_spatial_detector_host": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "hpx_stream_bits": (c_int, [c_u64, c_u32, c_u32, c_i64, c_vp, c_vp]),
+    "hpx_realization_normals": (c_int, [c_u64, c_u32, c_i64, c_vp, c_vp]),
 }
 
 _lib: ctypes.CDLL | None = None

@@ -21,6 +21,7 @@
 
 #include "hpx_common.hpp"
 #include "hpx_detect.hpp"
+#include "hpx_eig.hpp"
 #include "hpx_fading.hpp"
 #include "hpx_fft.hpp"
 #include "hpx_mapping.hpp"
@@ -56,7 +57,7 @@ struct BatchArgs {
   uint64_t seed;
   int64_t first_drop;
   // LDS byte offsets
-  int off_x, off_coef, off_abar, off_tw, off_bits, off_cons, off_s, off_red, off_tmp_sos;
+  int off_x, off_coef, off_abar, off_tw, off_bits, off_cons, off_s, off_det, off_red, off_tmp_sos, off_tmp_s64;
 };
 
 // bps bits of symbol q from an LSB-first word window (bit i of the result = i-th bit)
@@ -80,6 +81,10 @@ __global__ void __launch_bounds__(kBatchMaxThreads) batch_kernel(BatchArgs a) {
   int* red = reinterpret_cast<int*>(lds + a.off_red);
   double* uni = reinterpret_cast<double*>(lds + a.off_x);        // prologue only
   double* sos = reinterpret_cast<double*>(lds + a.off_tmp_sos);  // prologue only
+  double* s64 = reinterpret_cast<double*>(lds + a.off_tmp_s64);  // prologue only: S in float64
+  float2* det_v = reinterpret_cast<float2*>(lds + a.off_det);     // [NT][NT] eigenvectors of S^H S
+  float2* det_a = det_v + NT * NT;                                 // [NT][NR] V^H S^H
+  float* det_l = reinterpret_cast<float*>(det_a + NT * NR);        // [NT] eigenvalues
 
   const int tid = threadIdx.x;
   const int nthreads = blockDim.x;
@@ -117,6 +122,8 @@ __global__ void __launch_bounds__(kBatchMaxThreads) batch_kernel(BatchArgs a) {
       sincospi(2.0 * uni[10 * r + t], &si, &sr);
     }
     smat[e] = make_float2(static_cast<float>(sr), static_cast<float>(si));
+    s64[2 * e] = sr;
+    s64[2 * e + 1] = si;
   }
   // SoS table (fading.py:326-342, angles/phases :492-499): NLOS terms, then LOS
   {
@@ -153,6 +160,21 @@ __global__ void __launch_bounds__(kBatchMaxThreads) batch_kernel(BatchArgs a) {
 
   // Taylor coefficients of every (path, block) and word means of every (path, word)
   const TaylorBlocks tb{a.log2b};
+  if (tid == nthreads - 1) {
+    // per-drop detector of the rank-one-in-frequency ideal-CSI channel, on an LDS workspace
+    double* lam = s64 + 2 * NR * NT;
+    double* v = lam + NT;
+    double* am = v + 2 * NT * NT;
+    spatial_detector(NR, NT, s64, lam, v, am, am + 2 * NT * NR);
+    for (int i = 0; i < NT * NT; ++i) det_v[i] = make_float2(static_cast<float>(v[2 * i]), static_cast<float>(v[2 * i + 1]));
+    for (int i = 0; i < NT * NR; ++i)
+      det_a[i] = make_float2(static_cast<float>(am[2 * i]), static_cast<float>(am[2 * i + 1]));
+    double lmax = 0.0;
+    for (int i = 0; i < NT; ++i) lmax = fmax(lmax, lam[i]);
+    // numpy.linalg.pinv's cut: singular values below max(M, N) eps sigma_max are dropped
+    const double rt = static_cast<double>(NR > NT ? NR : NT) * 2.220446049250313e-16;
+    for (int i = 0; i < NT; ++i) det_l[i] = lam[i] > rt * rt * lmax ? static_cast<float>(lam[i]) : 0.0f;
+  }
   for (int e = tid; e < L * a.nb; e += nthreads) {
     const int p = e / a.nb, b = e - p * a.nb;
     taylor_coeffs(sos + 3 * p * a.terms, a.terms, tb.centre(b), tb.half_width(), coef + e * kTaylorCoeffs);
@@ -295,19 +317,30 @@ __global__ void __launch_bounds__(kBatchMaxThreads) batch_kernel(BatchArgs a) {
         if (d < 0 || d >= a.keep_below) continue;
         c = cfma(abar[p * a.words + w], tw[(bin * d) & (n - 1)], c);
       }
-      float2 xh[NT];
+      // W = V diag(conj(c) / (|c|^2 lambda_i + s2)) V^H S^H applied to y
       const float2* ycol = xbuf + x0 + bin;
-      if (NR == 1) {
-        const float2 yv = cscale(ycol[0], rx_scale);
-        float2 hs = make_float2(0.f, 0.f);
+      float2 yv[NR];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) hs = cadd(hs, cmul(smat[t], c));
-        const float inv = 1.0f / (cabs2(hs) + s2);
-        xh[0] = cscale(cmulc(hs, yv), inv);
-      } else {
-        detect_rows<NT>(
-            NR, [&](int r, int t) { return cmul(smat[r * NT + t], c); },
-            [&](int r) { return cscale(ycol[r * a.xs], rx_scale); }, s2, xh);
+      for (int r = 0; r < NR; ++r) yv[r] = ycol[r * a.xs];
+      const float c2 = cabs2(c);
+      float2 u[NT];
+#pragma unroll
+      for (int i = 0; i < NT; ++i) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc = cfma(det_a[i * NR + r], yv[r], acc);
+        const float den = fmaf(c2, det_l[i], s2);
+        // rx_scale folds the receiver's 1/sqrt(N); a zero gain realises pinv's dropped singular values
+        const float g = (det_l[i] > 0.0f || s2 > 0.0f) && den > 0.0f ? rx_scale / den : 0.0f;
+        u[i] = cscale(cmulc(c, acc), g);
+      }
+      float2 xh[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int i = 0; i < NT; ++i) acc = cfma(det_v[t * NT + i], u[i], acc);
+        xh[t] = acc;
       }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
@@ -420,9 +453,13 @@ int make_plan(const hpx_link* l, Plan* out) {
   a.off_x = 0;
   const int64_t xbytes = static_cast<int64_t>(rows) * a.xs * sizeof(float2);
   const int64_t uni_bytes = align16(static_cast<int64_t>(a.nnorm) * sizeof(double));
-  const int64_t sos_bytes = static_cast<int64_t>(a.paths) * a.terms * 3 * sizeof(double);
-  o += align16(std::max(xbytes, uni_bytes + sos_bytes));
+  const int64_t sos_bytes = align16(static_cast<int64_t>(a.paths) * a.terms * 3 * sizeof(double));
+  // S (float64), then the detector's outputs and Jacobi workspace
+  const int64_t s64_bytes =
+      static_cast<int64_t>(2 * nr * nt + nt + 2 * nt * nt + 2 * nt * nr + kEigWork) * sizeof(double);
+  o += align16(std::max(xbytes, uni_bytes + sos_bytes + s64_bytes));
   a.off_tmp_sos = static_cast<int>(uni_bytes);
+  a.off_tmp_s64 = static_cast<int>(uni_bytes + sos_bytes);
   a.off_coef = static_cast<int>(o);
   o += align16(static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs * sizeof(float2));
   a.off_abar = static_cast<int>(o);
@@ -435,6 +472,8 @@ int make_plan(const hpx_link* l, Plan* out) {
   o += align16(static_cast<int64_t>(a.order) * sizeof(float2));
   a.off_s = static_cast<int>(o);
   o += align16(static_cast<int64_t>(nr) * nt * sizeof(float2));
+  a.off_det = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(nt * nt + nt * nr) * sizeof(float2) + nt * sizeof(float));
   a.off_red = static_cast<int>(o);
   o += 16;
   out->lds = o;

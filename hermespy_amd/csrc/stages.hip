@@ -12,6 +12,7 @@
 
 #include "hpx_common.hpp"
 #include "hpx_detect.hpp"
+#include "hpx_eig.hpp"
 #include "hpx_fading.hpp"
 #include "hpx_fft.hpp"
 #include "hpx_mapping.hpp"
@@ -540,4 +541,59 @@ extern "C" int hpx_count_errors(const uint8_t* a, const uint8_t* b, int64_t num_
   const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((work + 255) / 256, 4096));
   hipLaunchKernelGGL(count_errors_kernel, dim3(blocks), dim3(256), 0, HPX_STREAM(stream), a, b, num_bits, errors);
   return check_launch("hpx_count_errors");
+}
+
+// ------------------------------------------------------------------ per-drop detector (host twin)
+extern "C" int hpx_spatial_detector_host(int num_rx, int num_tx, const hpx_c128* spatial_host, double* lambda_host,
+                                         hpx_c128* v_host, hpx_c128* a_host) {
+  HPX_REQUIRE(spatial_host && lambda_host && v_host && a_host, "hpx_spatial_detector_host: null pointer");
+  HPX_REQUIRE(num_tx >= 1 && num_tx <= kEigMax && num_rx >= 1 && num_rx <= kEigMax,
+              "hpx_spatial_detector_host: sizes must be in [1, %d]", kEigMax);
+  double work[kEigWork];
+  spatial_detector(num_rx, num_tx, reinterpret_cast<const double*>(spatial_host), lambda_host,
+                   reinterpret_cast<double*>(v_host), reinterpret_cast<double*>(a_host), work);
+  return HPX_OK;
+}
+
+// ------------------------------------------------------------------ random streams
+__global__ void __launch_bounds__(256) stream_bits_kernel(uint64_t seed, uint32_t drop, uint32_t stream_id,
+                                                          int64_t nbits, uint8_t* __restrict__ bits) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;  // one Philox call = 128 bits
+  if (c * 128 >= nbits) return;
+  const uint4 w = stream_call(seed, drop, stream_id, static_cast<uint32_t>(c));
+  const uint32_t comp[4] = {w.x, w.y, w.z, w.w};
+  for (int k = 0; k < 128; ++k) {
+    const int64_t i = c * 128 + k;
+    if (i < nbits) bits[i] = static_cast<uint8_t>((comp[k >> 5] >> (k & 31)) & 1u);
+  }
+}
+
+extern "C" int hpx_stream_bits(uint64_t seed, uint32_t drop, uint32_t stream_id, int64_t num_bits, uint8_t* bits,
+                               void* stream) {
+  HPX_REQUIRE(num_bits >= 0 && num_bits <= (int64_t(1) << 38), "hpx_stream_bits: invalid length");
+  if (num_bits == 0) return HPX_OK;
+  HPX_REQUIRE(bits, "hpx_stream_bits: null pointer");
+  const int64_t calls = (num_bits + 127) / 128;
+  hipLaunchKernelGGL(stream_bits_kernel, dim3(static_cast<unsigned>((calls + 255) / 256)), dim3(256), 0,
+                     HPX_STREAM(stream), seed, drop, stream_id, num_bits, bits);
+  return check_launch("hpx_stream_bits");
+}
+
+__global__ void __launch_bounds__(256) realization_normals_kernel(uint64_t seed, uint32_t drop, int64_t count,
+                                                                  double* __restrict__ out) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (2 * c >= count) return;
+  const double2 z = normal_pair_f64(stream_call(seed, drop, HPX_STREAM_REALIZATION, static_cast<uint32_t>(c)));
+  out[2 * c] = z.x;
+  if (2 * c + 1 < count) out[2 * c + 1] = z.y;
+}
+
+extern "C" int hpx_realization_normals(uint64_t seed, uint32_t drop, int64_t count, double* normals, void* stream) {
+  HPX_REQUIRE(count >= 0 && count <= (int64_t(1) << 32), "hpx_realization_normals: invalid count");
+  if (count == 0) return HPX_OK;
+  HPX_REQUIRE(normals, "hpx_realization_normals: null pointer");
+  const int64_t calls = (count + 1) / 2;
+  hipLaunchKernelGGL(realization_normals_kernel, dim3(static_cast<unsigned>((calls + 255) / 256)), dim3(256), 0,
+                     HPX_STREAM(stream), seed, drop, count, normals);
+  return check_launch("hpx_realization_normals");
 }

@@ -1,0 +1,307 @@
+"""OFDM modem of the drop path with the reference's API, computing on the GPU.
+
+Mirrors ``hermespy/modem/waveforms/orthogonal/waveform.py`` (``ElementType``,
+``PrefixType``, ``GridElement`` :65-90, ``GridResource`` :102-222,
+``SymbolSection`` :463-650, ``OrthogonalWaveform`` :950-1364),
+``hermespy/modem/waveforms/orthogonal/ofdm.py`` (``OFDMWaveform`` :33-161),
+``hermespy/modem/tools/psk_qam_mapping.py`` (``PskQamMapping`` :20-259),
+``hermespy/modem/waveform.py`` (``ZeroForcingChannelEqualization`` :235-250),
+``hermespy/simulation/modem/channel_estimation.py`` (``OFDMIdealChannelEstimation``
+:182-295) and ``hermespy/modem/evaluators.py`` (``BitErrorEvaluator`` :236-270).
+
+Arrays are torch tensors on the device (complex64 samples, uint8 bits).  The
+device path covers oversampling factor 1 and one cyclic-prefix length per frame.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from enum import Enum
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+class ElementType(Enum):
+    REFERENCE = 0
+    DATA = 1
+    NULL = 2
+
+
+class PrefixType(Enum):
+    CYCLIC = 0
+    ZEROPAD = 1
+    NONE = 2
+
+
+class GridElement:
+    def __init__(self, type: ElementType | str, repetitions: int = 1) -> None:
+        self.type = ElementType[type] if isinstance(type, str) else type
+        self.repetitions = int(repetitions)
+
+
+class GridResource:
+    """One frequency-domain resource pattern (``waveform.py:102-222``)."""
+
+    def __init__(self, repetitions: int = 1, prefix_type: PrefixType | str = PrefixType.CYCLIC,
+                 prefix_ratio: float = 0.0, elements: list[GridElement] | None = None) -> None:
+        if repetitions < 1:
+            raise ValueError("Number of frame resource repetitions must be greater or equal to one")
+        if prefix_ratio < 0.0:
+            raise ValueError(f"Cyclic prefix ratio must be non-negative (not {prefix_ratio})")
+        self.repetitions = int(repetitions)
+        self.prefix_type = PrefixType[prefix_type] if isinstance(prefix_type, str) else prefix_type
+        self.prefix_ratio = float(prefix_ratio)
+        self.elements = elements if elements is not None else []
+
+    @property
+    def num_subcarriers(self) -> int:
+        return self.repetitions * sum(e.repetitions for e in self.elements)
+
+    @property
+    def mask(self) -> np.ndarray:
+        """(len(ElementType), num_subcarriers) boolean mask (``waveform.py:204-222``)."""
+
+        m = np.zeros((len(ElementType), sum(e.repetitions for e in self.elements)), dtype=bool)
+        i = 0
+        for e in self.elements:
+            m[e.type.value, i:i + e.repetitions] = True
+            i += e.repetitions
+        return np.tile(m, (1, self.repetitions))
+
+
+class SymbolSection:
+    """Time-domain section of words (``waveform.py:463-650``)."""
+
+    def __init__(self, num_repetitions: int = 1, pattern: list[int] | None = None, sample_offset: int = 0) -> None:
+        if num_repetitions < 1:
+            raise ValueError("OFDM frame number of repetitions must be greater or equal to one")
+        if sample_offset != 0:
+            raise ValueError("sample offsets are not supported on the device path")
+        self.num_repetitions = int(num_repetitions)
+        self.pattern = pattern if pattern is not None else []
+        self.sample_offset = sample_offset
+
+    @property
+    def num_words(self) -> int:
+        return self.num_repetitions * len(self.pattern)
+
+
+class PskQamMapping:
+    """3GPP Gray mapping (``psk_qam_mapping.py:20-259``) on the device."""
+
+    def __init__(self, modulation_order: int, is_complex: bool = True) -> None:
+        if modulation_order <= 0 or (modulation_order & (modulation_order - 1)) != 0:
+            raise ValueError("modulation_order must be a power of two")
+        self.modulation_order = int(modulation_order)
+        self.is_complex = bool(is_complex)
+        self.bits_per_symbol = int(np.log2(modulation_order))
+
+    def get_symbols(self, bits: torch.Tensor) -> torch.Tensor:
+        n = bits.numel() // self.bits_per_symbol
+        bits = bits.to(torch.uint8).contiguous()
+        out = torch.empty(n, dtype=torch.complex64, device=bits.device)
+        _lib.check(_lib.load().hpx_map_symbols(_lib.ptr(bits), n, self.modulation_order, int(self.is_complex),
+                                               _lib.ptr(out), _lib.stream_handle()))
+        return out
+
+    def detect_bits(self, symbols: torch.Tensor) -> torch.Tensor:
+        symbols = symbols.to(torch.complex64).contiguous().reshape(-1)
+        out = torch.empty(symbols.numel() * self.bits_per_symbol, dtype=torch.uint8, device=symbols.device)
+        _lib.check(_lib.load().hpx_demap_hard(_lib.ptr(symbols), symbols.numel(), self.modulation_order,
+                                              int(self.is_complex), _lib.ptr(out), _lib.stream_handle()))
+        return out
+
+    def get_mapping(self) -> np.ndarray:
+        """Constellation in label order (``psk_qam_mapping.py:499-524``), float64 as the reference rounds it."""
+
+        pts = np.zeros(2 * self.modulation_order)
+        _lib.check(_lib.load().hpx_constellation_host(self.modulation_order, int(self.is_complex), pts.ctypes.data))
+        return pts[0::2] + 1j * pts[1::2]
+
+
+class OFDMWaveform:
+    """OFDM frame of ``SymbolSection`` words (``ofdm.py:33-161``, ``waveform.py:950-1364``)."""
+
+    def __init__(self, grid_resources: list[GridResource], grid_structure: list[SymbolSection],
+                 num_subcarriers: int = 1024, dc_suppression: bool = True, modulation_order: int = 16,
+                 oversampling_factor: int = 1) -> None:
+        if num_subcarriers < 1:
+            raise ValueError("Number of subcarriers must be greater or equal to one")
+        if oversampling_factor != 1:
+            raise ValueError("the device path runs at oversampling factor 1")
+        self.num_subcarriers = int(num_subcarriers)
+        self.grid_resources = grid_resources
+        self.grid_structure = grid_structure
+        self.dc_suppression = bool(dc_suppression)
+        self.mapping = PskQamMapping(modulation_order)
+        prefixes = set()
+        for sec in grid_structure:
+            for r in sec.pattern:
+                res = grid_resources[r]
+                cp = int(self.num_subcarriers * 1 * res.prefix_ratio)  # waveform.py:538
+                if cp > 0 and res.prefix_type == PrefixType.ZEROPAD:
+                    raise ValueError("zero-padding prefixes are not supported on the device path")
+                prefixes.add(cp if res.prefix_type != PrefixType.NONE else 0)
+        if len(prefixes) > 1:
+            raise ValueError("the device path needs one prefix length per frame")
+        self.prefix = prefixes.pop() if prefixes else 0
+        self._mask = self.resource_mask()
+        self._data_index = torch.as_tensor(np.flatnonzero(self._mask[ElementType.DATA.value].ravel()))
+
+    @property
+    def modulation_order(self) -> int:
+        return self.mapping.modulation_order
+
+    @property
+    def bits_per_symbol(self) -> int:
+        return self.mapping.bits_per_symbol
+
+    @property
+    def words_per_frame(self) -> int:
+        return sum(s.num_words for s in self.grid_structure)
+
+    def resource_mask(self) -> np.ndarray:
+        """(len(ElementType), words, num_subcarriers) (``waveform.py:1194-1211``)."""
+
+        mask = np.zeros((len(ElementType), self.words_per_frame, self.num_subcarriers), dtype=bool)
+        w = 0
+        for sec in self.grid_structure:
+            for _ in range(sec.num_repetitions):
+                for r in sec.pattern:
+                    m = self.grid_resources[r].mask
+                    mask[:, w, :m.shape[1]] = m
+                    w += 1
+        return mask
+
+    @property
+    def num_data_symbols(self) -> int:
+        return int(self._mask[ElementType.DATA.value].sum())
+
+    def bits_per_frame(self) -> int:
+        return self.num_data_symbols * self.bits_per_symbol
+
+    def samples_per_frame(self, bandwidth: float = 0.0, oversampling_factor: int = 1) -> int:
+        return self.words_per_frame * (self.num_subcarriers + self.prefix)
+
+    # ------------------------------------------------------------ Tx
+    def map(self, bits: torch.Tensor) -> torch.Tensor:
+        return self.mapping.get_symbols(bits)
+
+    def place(self, symbols: torch.Tensor) -> torch.Tensor:
+        """Data symbols (..., num_data_symbols) -> grids (..., words, N); reference symbols are zero."""
+
+        if symbols.shape[-1] != self.num_data_symbols:
+            raise ValueError("Number of provided data symbols does not match the number of symbols in the frame "
+                             f"({symbols.shape[-1]} != {self.num_data_symbols})")
+        lead = symbols.shape[:-1]
+        grid = torch.zeros(lead + (self.words_per_frame * self.num_subcarriers,), dtype=torch.complex64,
+                           device=symbols.device)
+        grid[..., self._data_index.to(symbols.device)] = symbols
+        return grid.reshape(lead + (self.words_per_frame, self.num_subcarriers))
+
+    def modulate(self, grid: torch.Tensor) -> torch.Tensor:
+        """(streams, words, N) grids -> (streams, samples_per_frame) frames."""
+
+        grid = grid.to(torch.complex64).contiguous()
+        streams = grid.shape[0] if grid.ndim == 3 else 1
+        m = self.samples_per_frame()
+        out = torch.empty((streams, m), dtype=torch.complex64, device=grid.device)
+        _lib.check(_lib.load().hpx_ofdm_modulate(_lib.ptr(grid), streams, self.words_per_frame, self.num_subcarriers,
+                                                 self.prefix, int(self.dc_suppression), _lib.ptr(out), m,
+                                                 _lib.stream_handle()))
+        return out
+
+    # ------------------------------------------------------------ Rx
+    def demodulate(self, signal: torch.Tensor) -> torch.Tensor:
+        """(streams, >= samples_per_frame) -> (streams, words, N) (frame at offset 0, ``waveform.py:90-99``)."""
+
+        signal = signal.to(torch.complex64)
+        if signal.ndim == 1:
+            signal = signal[None]
+        m = self.samples_per_frame()
+        if signal.shape[1] < m:
+            signal = torch.nn.functional.pad(signal, (0, m - signal.shape[1]))
+        signal = signal.contiguous()
+        out = torch.empty((signal.shape[0], self.words_per_frame, self.num_subcarriers), dtype=torch.complex64,
+                          device=signal.device)
+        _lib.check(_lib.load().hpx_ofdm_demodulate(_lib.ptr(signal), signal.shape[0], signal.shape[1],
+                                                   self.words_per_frame, self.num_subcarriers, self.prefix,
+                                                   int(self.dc_suppression), _lib.ptr(out), _lib.stream_handle()))
+        return out
+
+    def pick(self, grid: torch.Tensor) -> torch.Tensor:
+        """Data resource elements of (..., words, N) grids -> (..., num_data_symbols)."""
+
+        flat = grid.reshape(grid.shape[:-2] + (-1,))
+        return flat[..., self._data_index.to(grid.device)]
+
+    def unmap(self, symbols: torch.Tensor) -> torch.Tensor:
+        return self.mapping.detect_bits(symbols)
+
+
+class OFDMIdealChannelEstimation:
+    """Ideal CSI of a fading sample (``channel_estimation.py:182-295``): (Nrx, Ntx, words, N)."""
+
+    def __init__(self, waveform: OFDMWaveform) -> None:
+        self.waveform = waveform
+
+    def estimate_channel(self, sample) -> torch.Tensor:
+        wf = self.waveform
+        d = sample._device_tables()
+        nrx, ntx = sample.num_receive_antennas, sample.num_transmit_antennas
+        out = torch.empty((nrx, ntx, wf.words_per_frame, wf.num_subcarriers), dtype=torch.complex64,
+                          device=sample.device)
+        m = wf.samples_per_frame()
+        num_taps = min(1 + round(float(sample.delay_profile.max()) * sample.bandwidth), m)  # fading.py:351
+        _lib.check(_lib.load().hpx_ideal_csi(_lib.ptr(d["sos"]), sample.num_paths, sample.num_sinusoids + 1,
+                                             _lib.ptr(d["delays"]), num_taps, _lib.ptr(d["spatial"]), nrx, ntx,
+                                             wf.words_per_frame, wf.num_subcarriers, wf.prefix,
+                                             int(wf.dc_suppression), _lib.ptr(out), _lib.stream_handle()))
+        return out
+
+
+def _equalize(y: torch.Tensor, h: torch.Tensor, mode: int, noise_variance: float) -> torch.Tensor:
+    y = y.to(torch.complex64).contiguous()
+    h = h.to(torch.complex64).contiguous()
+    nrx, ntx, k = h.shape
+    out = torch.empty((ntx, k), dtype=torch.complex64, device=y.device)
+    _lib.check(_lib.load().hpx_equalize(_lib.ptr(y), _lib.ptr(h), nrx, ntx, k, mode, float(noise_variance),
+                                        _lib.ptr(out), _lib.stream_handle()))
+    return out
+
+
+class ZeroForcingChannelEqualization:
+    """``waveform.py:235-250``: y (Nrx, K), h (Nrx, Ntx, K) -> (Ntx, K)."""
+
+    def equalize_channel(self, y: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
+        return _equalize(y, h, _lib.HPX_DETECT_ZF, 0.0)
+
+
+class MMSEChannelEqualization:
+    """Linear MMSE (H^H H + sigma^2 I)^{-1} H^H y; no reference counterpart for OFDM."""
+
+    def __init__(self, noise_variance: float = 0.0) -> None:
+        self.noise_variance = float(noise_variance)
+
+    def equalize_channel(self, y: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
+        return _equalize(y, h, _lib.HPX_DETECT_MMSE, self.noise_variance)
+
+
+class BitErrorEvaluator:
+    """``evaluators.py:236-270``: bit errors between transmitted and received bits."""
+
+    def evaluate(self, transmitted: torch.Tensor, received: torch.Tensor, out: torch.Tensor | None = None):
+        n = max(transmitted.numel(), received.numel())
+        a = transmitted.reshape(-1).to(torch.uint8)
+        b = received.reshape(-1).to(torch.uint8)
+        if a.numel() < n:  # zero-padding of the shorter sequence (evaluators.py:246-252)
+            a = torch.nn.functional.pad(a, (0, n - a.numel()))
+        if b.numel() < n:
+            b = torch.nn.functional.pad(b, (0, n - b.numel()))
+        a, b = a.contiguous(), b.contiguous()
+        acc = torch.zeros(1, dtype=torch.int64, device=a.device) if out is None else out
+        _lib.check(_lib.load().hpx_count_errors(_lib.ptr(a), _lib.ptr(b), n, _lib.ptr(acc), _lib.stream_handle()))
+        return acc

@@ -117,3 +117,74 @@ class BatchedSimulation:
         per_err = np.bincount(idx, weights=err, minlength=snr.size).astype(np.int64)
         per_drops = np.bincount(idx, minlength=snr.size).astype(np.int64)
         return BatchResult(snr, per_err, per_drops, self.link.bits_per_drop)
+
+
+class SingleDropLink:
+    """The plain GPU path: one drop at a time through the stage entry points.
+
+    Per drop: realisation normals (device) -> host realisation sample (the
+    reference's ``MultipathFadingRealization._sample``) -> bits -> mapping ->
+    OFDM modulation -> TDL propagation -> AWGN -> demodulation -> ideal CSI ->
+    equalisation -> hard demapping -> bit error count, each a separate launch,
+    consuming the same Philox streams as the fused batch kernel.
+    """
+
+    def __init__(self, link: LinkConfig, seed: int = 20261202, device: str = "cuda") -> None:
+        from .channel import CustomAntennaCorrelation, MultipathFadingChannel
+        from .modem import (BitErrorEvaluator, GridElement, GridResource, MMSEChannelEqualization, OFDMIdealChannelEstimation,
+                            OFDMWaveform, SymbolSection, ZeroForcingChannelEqualization)
+
+        self.link = link
+        self.seed = int(seed)
+        self.device = torch.device(device)
+        corr = CustomAntennaCorrelation(link.corr_rx) if link.corr_rx is not None else None
+        if link.corr_rx is not None and not np.allclose(link.corr_rx[:link.num_tx, :link.num_tx], link.corr_tx):
+            raise ValueError("SingleDropLink models one correlation matrix for both ends")
+        self.channel = MultipathFadingChannel(link.delays, link.power_profile, link.rice_factors,
+                                              num_sinusoids=link.num_sinusoids, doppler_frequency=link.doppler,
+                                              los_doppler_frequency=link.los_doppler, antenna_correlation=corr,
+                                              gain=link.gain, los_angle=0.0)
+        res = GridResource(1, "CYCLIC", link.prefix_ratio, [GridElement("DATA", link.num_subcarriers)])
+        self.waveform = OFDMWaveform([res], [SymbolSection(link.words, [0])], link.num_subcarriers,
+                                     link.dc_suppression, link.modulation_order)
+        self.estimator = OFDMIdealChannelEstimation(self.waveform)
+        self.zf = ZeroForcingChannelEqualization()
+        self.mmse = MMSEChannelEqualization()
+        self.ber = BitErrorEvaluator()
+
+    def run_drop(self, drop: int, snr_db: float, errors: torch.Tensor | None = None,
+                 return_equalized: bool = False):
+        from .channel import MultipathFadingRealization
+
+        lk, lib, st = self.link, _lib.load(), _lib.stream_handle()
+        dev = self.device
+        # channel realisation: the drop's normals, sampled on the host like the reference
+        z = torch.empty(self.channel.num_scalars, dtype=torch.float64, device=dev)
+        _lib.check(lib.hpx_realization_normals(self.seed, drop, z.numel(), _lib.ptr(z), st))
+        sample = MultipathFadingRealization(z.cpu().numpy(), self.channel).sample(lk.num_tx, lk.num_rx, lk.bandwidth, dev)
+        # transmitter
+        nbits = self.waveform.bits_per_frame()
+        bits = torch.empty((lk.num_tx, nbits), dtype=torch.uint8, device=dev)
+        for t in range(lk.num_tx):
+            _lib.check(lib.hpx_stream_bits(self.seed, drop, _lib.STREAM_BITS + t, nbits, _lib.ptr(bits[t]), st))
+        symbols = self.waveform.map(bits.reshape(-1)).reshape(lk.num_tx, -1)
+        x = self.waveform.modulate(self.waveform.place(symbols))
+        if lk.device_power != 1.0:
+            x = x * float(np.sqrt(lk.device_power))
+        # channel and noise
+        y = sample.propagate(x)
+        p = float(lk.noise_power(snr_db))
+        _lib.check(lib.hpx_awgn(_lib.ptr(y), y.shape[0], y.shape[1], y.shape[1], self.seed, drop, p, st))
+        # receiver
+        grid = self.waveform.demodulate(y)
+        h = self.estimator.estimate_channel(sample)
+        yd = self.waveform.pick(grid)
+        hd = self.waveform.pick(h)
+        if lk.detector == "zf":
+            xh = self.zf.equalize_channel(yd, hd)
+        else:
+            self.mmse.noise_variance = p
+            xh = self.mmse.equalize_channel(yd, hd)
+        rx_bits = self.waveform.unmap(xh)
+        acc = self.ber.evaluate(bits, rx_bits, errors)
+        return (acc, xh) if return_equalized else acc

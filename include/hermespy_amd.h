@@ -90,6 +90,17 @@ int hpx_tdl_propagate(const hpx_c64* x, int num_tx, int64_t ld_x, int64_t num_sa
 int hpx_awgn(hpx_c64* signal, int num_streams, int64_t ld, int64_t num_samples, uint64_t seed, uint32_t drop,
              double power, void* stream);
 
+/* Data bits of transmit stream `stream_id` - HPX_STREAM_BITS of `drop` (one uint8 0/1 per
+ * bit; bit k = bit k % 32 of word k / 32).  Replaces RandomBitsSource.generate_bits,
+ * hermespy/modem/bits_source.py:62-63 (a counter-based stream instead of PCG64). */
+int hpx_stream_bits(uint64_t seed, uint32_t drop, uint32_t stream_id, int64_t num_bits, uint8_t* bits,
+                    void* stream);
+
+/* float64 standard normals of one drop's channel realisation (Box-Muller on
+ * stream HPX_STREAM_REALIZATION).  Replaces the static branch of
+ * ConsistentGenerator.realize, hermespy/channel/consistent.py:431-432. */
+int hpx_realization_normals(uint64_t seed, uint32_t drop, int64_t count, double* normals, void* stream);
+
 /* Bits (one uint8 0/1 per bit) -> symbols, 3GPP Gray mapping.
  * Replaces PskQamMapping.get_symbols, hermespy/modem/tools/psk_qam_mapping.py:104-169. */
 int hpx_map_symbols(const uint8_t* bits, int64_t num_symbols, int modulation_order, int is_complex,
@@ -177,6 +188,16 @@ int hpx_run_batch(const hpx_link* link_host, const int32_t* delays, const double
                   const hpx_c128* corr_rx, const hpx_c128* corr_tx, uint64_t seed, int64_t first_drop,
                   int64_t num_drops, const double* noise_power, int32_t* errors, hpx_c64* equalized,
                   void* stream);
+
+/* Host twin of the per-drop detector hpx_run_batch builds for ideal CSI, where every
+ * resource element's channel is H_k = c_k S: S^H S = V diag(lambda) V^H (cyclic
+ * Jacobi, float64) and A = V^H S^H, so that the MMSE/ZF filter of element k is
+ * V diag(conj(c_k) / (|c_k|^2 lambda_i + s2)) A.  S: [num_rx][num_tx]; outputs
+ * lambda [num_tx], V [num_tx][num_tx], A [num_tx][num_rx]; sizes <= 8.
+ * No reference counterpart (the reference applies numpy.linalg.pinv per element,
+ * hermespy/modem/waveform.py:247). */
+int hpx_spatial_detector_host(int num_rx, int num_tx, const hpx_c128* spatial_host, double* lambda_host,
+                              hpx_c128* v_host, hpx_c128* a_host);
 
 /* Dynamic LDS bytes one hpx_run_batch workgroup needs for `link` (0 if unsupported). */
 int64_t hpx_run_batch_lds_bytes(const hpx_link* link_host);

@@ -75,3 +75,36 @@ def test_host_demap_bit_exact(native_lib, order, is_complex):
     assert native_lib.hpx_demap_hard_host(xx.ctypes.data, x.size, order, is_complex, bits.ctypes.data) == 0
     exp = mapping.detect_bits(x if is_complex else x.real + 0j, order, bool(is_complex)).astype(np.uint8)
     assert np.array_equal(bits, exp)
+
+
+@pytest.mark.parametrize("nr,nt", [(1, 1), (2, 2), (4, 4), (8, 8), (4, 2), (8, 4)])
+def test_host_spatial_detector(native_lib, nr, nt):
+    """Jacobi eigen-decomposition of S^H S against numpy.linalg.eigh, and the per-element
+    filter V diag(conj(c) / (|c|^2 lambda + s2)) V^H S^H against the MMSE closed form."""
+
+    from oracle import equalize
+
+    rng = np.random.default_rng(nr * 8 + nt)
+    r = 0.5 ** np.abs(np.subtract.outer(np.arange(nr), np.arange(nr)))
+    rt = 0.5 ** np.abs(np.subtract.outer(np.arange(nt), np.arange(nt)))
+    s = r @ np.exp(2j * np.pi * rng.random((nr, nt))) @ rt
+    lam = np.zeros(nt)
+    v = np.zeros((nt, nt), np.complex128)
+    a = np.zeros((nt, nr), np.complex128)
+    sc = np.ascontiguousarray(s)
+    assert native_lib.hpx_spatial_detector_host(nr, nt, sc.ctypes.data, lam.ctypes.data, v.ctypes.data,
+                                                a.ctypes.data) == 0
+    g = s.conj().T @ s
+    np.testing.assert_allclose(np.sort(lam), np.linalg.eigvalsh(g), rtol=1e-10, atol=1e-12 * np.abs(g).max())
+    np.testing.assert_allclose(v @ np.diag(lam) @ v.conj().T, g, atol=1e-12 * np.abs(g).max())
+    np.testing.assert_allclose(v.conj().T @ v, np.eye(nt), atol=1e-12)
+    np.testing.assert_allclose(a, v.conj().T @ s.conj().T, atol=1e-12)
+    c = 0.7 - 0.2j
+    y = rng.standard_normal(nr) + 1j * rng.standard_normal(nr)
+    for s2 in (0.0, 0.1):
+        if s2 == 0.0 and nr < nt:
+            continue
+        x = v @ ((np.conj(c) / (abs(c) ** 2 * lam + s2)) * (a @ y))
+        h = (c * s)[:, :, None]
+        ref = equalize.mmse(y[:, None], h, s2)[:, 0] if s2 > 0 else equalize.zero_forcing(y[:, None], h)[:, 0]
+        np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12)
