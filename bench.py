@@ -95,6 +95,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain single-drop GPU path")
+    ap.add_argument("--no-tune", action="store_true", help="use the default workgroup size instead of the autotuner's")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (counter calibration for profiles)")
     args = ap.parse_args()
@@ -108,7 +109,9 @@ def main() -> int:
     from hermespy_amd.configs import mimo4x4_1024
     from hermespy_amd.simulation import BatchedLink
 
-    cfg = mimo4x4_1024()
+    from hermespy_amd.autotune import tuned
+
+    cfg = tuned(mimo4x4_1024(), verbose=True) if not args.no_tune else mimo4x4_1024()
     snr = np.asarray(cfg.snr_db)
     link = BatchedLink(cfg)
     dev = link.device
@@ -161,13 +164,14 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (Philox channel realisations, bits and noise drawn on the device)",
         "config": {"workload": cfg.name, "drops_per_step": n, "snr_db": snr.tolist(),
-                   "bits_per_drop": cfg.bits_per_drop, "detector": cfg.detector, "csi": "ideal"},
+                   "bits_per_drop": cfg.bits_per_drop, "detector": cfg.detector, "csi": "ideal",
+                   "workgroup_size": cfg.workgroup_size or "default"},
         "bits_per_s": drops_per_s * cfg.bits_per_drop,
         "wall_s_timed": wall,
         "ber_by_snr": [float(b) for b in ber],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "batch_kernel<4,4,1>", "flops_per_drop": algorithmic_flops_per_drop(cfg)},
+                     "kernel": "batch_kernel<4,4,P,T>", "flops_per_drop": algorithmic_flops_per_drop(cfg)},
     }
     if not args.no_plain:
         plain = plain_gpu_path(cfg)

@@ -50,6 +50,7 @@ class HpxLink(ctypes.Structure):
         ("dc_suppression", c_i32),
         ("bandwidth", c_dbl), ("doppler", c_dbl), ("los_doppler", c_dbl), ("gain", c_dbl),
         ("device_power", c_dbl),
+        ("workgroup_size", c_i32),
     ]
 
 

@@ -13,6 +13,7 @@ and the dense impulse response run in HIP through libhermespy_amd.
 from __future__ import annotations
 
 import ctypes
+from enum import Enum
 
 import numpy as np
 import torch
@@ -209,8 +210,14 @@ class MultipathFadingRealization:
         nlos_ph_u = u[o:o + L * N].reshape(L, N)
         spatial = np.exp(2j * np.pi * corr)[:num_receive_antennas, :num_transmit_antennas]
         if ch.antenna_correlation is not None:
-            spatial = (ch.antenna_correlation.sample_covariance(num_receive_antennas) @ spatial
-                       @ ch.antenna_correlation.sample_covariance(num_transmit_antennas))
+            corr = ch.antenna_correlation
+            if isinstance(corr, StandardAntennaCorrelation):
+                rrx = corr.covariance_for(num_receive_antennas, True)
+                rtx = corr.covariance_for(num_transmit_antennas, False)
+            else:
+                rrx = corr.sample_covariance(num_receive_antennas)
+                rtx = corr.sample_covariance(num_transmit_antennas)
+            spatial = rrx @ spatial @ rtx
         return MultipathFadingSample(ch.power_profile, ch.delays, 2 * np.pi * los_u, -np.pi + 2 * np.pi * nlos_ang_u,
                                      -np.pi + 2 * np.pi * los_ph_u, -np.pi + 2 * np.pi * nlos_ph_u, ch.los_gains,
                                      ch.nlos_gains, ch.los_doppler_frequency, ch.doppler_frequency, spatial, ch.gain,
@@ -305,3 +312,126 @@ class MultipathFadingChannel:
         x = _as_device_signal(signal, device)
         sample = self.realize().sample(int(x.shape[0]), num_receive_antennas, bandwidth, device)
         return sample.propagate(x)
+
+
+# ---------------------------------------------------------------------- parameterisations
+_TABLES: dict | None = None
+
+
+def _tables() -> dict:
+    global _TABLES
+    if _TABLES is None:
+        import json
+        from pathlib import Path
+
+        _TABLES = json.loads((Path(__file__).parent / "data" / "fading_tables.json").read_text())
+    return _TABLES
+
+
+class TDLType(Enum):
+    """5G TDL model types (``tdl.py:18-25``)."""
+
+    A = 0
+    B = 1
+    C = 2
+    D = 4
+    E = 5
+
+
+class TDL(MultipathFadingChannel):
+    """3GPP TR 38.901 TDL-A..E (``tdl.py:28-397``): normalised delays scaled by ``rms_delay``,
+    powers normalised to sum one; D and E carry a specular first tap (K 13.3 / 22, LOS Doppler 0.7)."""
+
+    def __init__(self, model_type: TDLType = TDLType.A, rms_delay: float = 0.0, **kwargs) -> None:
+        if rms_delay < 0.0:
+            raise ValueError("Root-Mean-Squared delay must be greater or equal to zero")
+        model_type = TDLType(model_type)
+        t = _tables()["TDL"][model_type.name]
+        los_doppler = kwargs.pop("los_doppler_frequency", None)
+        if t["los_doppler_frequency"] is not None:
+            if los_doppler is not None:
+                raise ValueError(f"Model type {model_type.name} does not support line of sight doppler frequency "
+                                 f"configuration")
+            los_doppler = t["los_doppler_frequency"]
+        p = 10.0 ** (np.asarray(t["power_db"], dtype=np.float64) / 10)
+        p /= np.sum(p)
+        k = np.zeros(p.size)
+        k[0] = t["rice_factor_first"]
+        self.model_type = model_type
+        self.rms_delay = float(rms_delay)
+        MultipathFadingChannel.__init__(self, rms_delay * np.asarray(t["normalized_delays"], dtype=np.float64), p, k,
+                                        los_doppler_frequency=los_doppler, **kwargs)
+
+
+class Cost259Type(Enum):
+    URBAN = 0
+    RURAL = 1
+    HILLY = 2
+
+
+class Cost259(MultipathFadingChannel):
+    """COST 259 typical urban / rural area / hilly terrain (``cost259.py:35-262``)."""
+
+    def __init__(self, model_type: Cost259Type = Cost259Type.URBAN, **kwargs) -> None:
+        model_type = Cost259Type(model_type)
+        t = _tables()["COST259"][model_type.name]
+        k = np.zeros(len(t["delays_s"]))
+        if t["los_first"]:
+            if kwargs.get("los_angle") is not None:
+                raise ValueError("Model type HILLY does not support line of sight angle configuration")
+            k[0] = np.inf
+            kwargs["los_angle"] = float(np.arccos(t["los_angle_cos"]))
+        p = 10.0 ** (np.asarray(t["power_db"], dtype=np.float64) / 10)
+        p /= np.sum(p)
+        self.model_type = model_type
+        MultipathFadingChannel.__init__(self, np.asarray(t["delays_s"], dtype=np.float64), p, k, **kwargs)
+
+
+class Exponential(MultipathFadingChannel):
+    """Exponential power delay profile (``exponential.py:19-117``): taps every ``tap_interval``,
+    decay alpha from the rms delay of a geometric profile, truncated below 1e-5."""
+
+    _TRUNCATION = 1e-5
+
+    def __init__(self, tap_interval: float, rms_delay: float, **kwargs) -> None:
+        if tap_interval <= 0.0:
+            raise ValueError("Tap interval must be greater than zero")
+        if rms_delay <= 0.0:
+            raise ValueError("Root-Mean-Squared delay must be greater than zero")
+        rms_norm = rms_delay / tap_interval
+        alpha = -2 * np.log((-1 + np.sqrt(1 + 4 * rms_norm ** 2)) / (2 * rms_norm))
+        taps = int(-np.ceil(np.log(self._TRUNCATION) / alpha))
+        self.tap_interval = float(tap_interval)
+        self.rms_delay = float(rms_delay)
+        MultipathFadingChannel.__init__(self, np.arange(taps + 1) * tap_interval, np.exp(-alpha * np.arange(taps + 1)),
+                                        np.zeros(taps + 1), **kwargs)
+
+
+class CorrelationType(Enum):
+    """3GPP correlation levels (base station, terminal) (``correlation.py:30-45``)."""
+
+    LOW = (0.0, 0.0)
+    MEDIUM = (0.3, 0.3)
+    MEDIUM_A = (0.3, 0.3874)
+    HIGH = (0.9, 0.9)
+
+
+class StandardAntennaCorrelation(AntennaCorrelation):
+    """3GPP standard covariance for 1, 2 or 4 antennas (``correlation.py:48-106``).  The transmitter
+    is taken as the base station and the receiver as the terminal, as in the reference."""
+
+    def __init__(self, correlation: CorrelationType | str) -> None:
+        self.correlation = CorrelationType[correlation] if isinstance(correlation, str) else CorrelationType(correlation)
+
+    def covariance_for(self, num_antennas: int, receive: bool) -> np.ndarray:
+        f = self.correlation.value[1 if receive else 0]
+        n = num_antennas
+        if n == 1:
+            return np.ones((1, 1), dtype=complex)
+        if n == 2:
+            return np.array([[1, f], [f, 1]], dtype=complex)
+        if n == 4:
+            a, b = f ** (1 / 9), f ** (4 / 9)
+            return np.array([[1, a, b, f], [a, 1, a, b], [b, a, 1, a], [f, b, a, 1]], dtype=complex)
+        raise RuntimeError(f"3GPP standard antenna covariance is only defined for 1, 2 and 4 antennas, device has "
+                           f"{n} antennas")

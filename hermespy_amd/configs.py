@@ -39,6 +39,7 @@ class LinkConfig:
     los_doppler: float | None = None
     name: str = "link"
     snr_db: tuple = field(default=(0.0,))
+    workgroup_size: int = 0  # threads per drop in the fused kernel (0: library default)
 
     def __post_init__(self) -> None:
         order = np.argsort(np.asarray(self.delays, dtype=np.float64))  # MultipathFadingChannel sorts, fading.py:707-711

@@ -68,8 +68,8 @@ __device__ __forceinline__ uint32_t symbol_bits(const uint32_t* bw, int q, int b
   return static_cast<uint32_t>(v >> (bit & 31)) & ((1u << bps) - 1u);
 }
 
-template <int NT, int NR, int PAIRS>
-__global__ void __launch_bounds__(kBatchMaxThreads) batch_kernel(BatchArgs a) {
+template <int NT, int NR, int PAIRS, int MAXT>
+__global__ void __launch_bounds__(MAXT) batch_kernel(BatchArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   float2* xbuf = reinterpret_cast<float2*>(lds + a.off_x);
   float2* coef = reinterpret_cast<float2*>(lds + a.off_coef);
@@ -79,6 +79,8 @@ __global__ void __launch_bounds__(kBatchMaxThreads) batch_kernel(BatchArgs a) {
   float2* cons = reinterpret_cast<float2*>(lds + a.off_cons);
   float2* smat = reinterpret_cast<float2*>(lds + a.off_s);
   int* red = reinterpret_cast<int*>(lds + a.off_red);
+  constexpr int kRows = NT > NR ? NT : NR;
+  float2* csirow = xbuf + kRows * a.xs + a.dx + a.cp;              // [N] per-word CSI over the taps
   double* uni = reinterpret_cast<double*>(lds + a.off_x);        // prologue only
   double* sos = reinterpret_cast<double*>(lds + a.off_tmp_sos);  // prologue only
   double* s64 = reinterpret_cast<double*>(lds + a.off_tmp_s64);  // prologue only: S in float64
@@ -216,17 +218,25 @@ __global__ void __launch_bounds__(kBatchMaxThreads) batch_kernel(BatchArgs a) {
       }
     }
     __syncthreads();
-    // map + place at the IFFT bins
+    // map + place at the IFFT bins; clear the CSI row (row kRows: tap means at their delays)
     for (int e = tid; e < NT * n; e += nthreads) {
       const int t = e / n, q = e - t * n;
       xbuf[t * a.xs + x0 + subcarrier_bin(q, n, a.dc)] = cons[symbol_bits(bits + t * a.bits_words, q, a.bps)];
     }
+    for (int q = tid; q < n; q += nthreads) csirow[q] = make_float2(0.f, 0.f);
     __syncthreads();
     lds_fft<+1>(xbuf + x0, NT, a.xs, a.log2n, tw);
-    // cyclic prefix
+    // cyclic prefix; word means of the taps at their delays (paths sharing a delay add up,
+    // taps beyond min(num_taps, N) are cut as fft(n=N) cuts them)
     for (int e = tid; e < NT * a.cp; e += nthreads) {
       const int t = e / a.cp, i = e - t * a.cp;
       xbuf[t * a.xs + a.dx + i] = xbuf[t * a.xs + a.dx + n + i];
+    }
+    if (tid == 0) {
+      for (int p = 0; p < L; ++p) {
+        const int d = a.delays[p];
+        if (d >= 0 && d < a.keep_below) csirow[d] = cadd(csirow[d], abar[p * a.words + w]);
+      }
     }
     __syncthreads();
 
@@ -306,17 +316,13 @@ __global__ void __launch_bounds__(kBatchMaxThreads) batch_kernel(BatchArgs a) {
       }
     }
     __syncthreads();
-    lds_fft<-1>(xbuf + x0, NR, a.xs, a.log2n, tw);
+    // receive FFT of every antenna and, in the same passes, the CSI row's FFT over the taps
+    lds_fft<-1>(xbuf + x0, kRows + 1, a.xs, a.log2n, tw);
 
     // ideal CSI, detection, demapping and bit errors per subcarrier
     for (int q = tid; q < n; q += nthreads) {
       const int bin = subcarrier_bin(q, n, a.dc);
-      float2 c = make_float2(0.f, 0.f);
-      for (int p = 0; p < L; ++p) {
-        const int d = a.delays[p];
-        if (d < 0 || d >= a.keep_below) continue;
-        c = cfma(abar[p * a.words + w], tw[(bin * d) & (n - 1)], c);
-      }
+      const float2 c = csirow[bin];
       // W = V diag(conj(c) / (|c|^2 lambda_i + s2)) V^H S^H applied to y
       const float2* ycol = xbuf + x0 + bin;
       float2 yv[NR];
@@ -433,11 +439,23 @@ int make_plan(const hpx_link* l, Plan* out) {
   a.keep_below = static_cast<int>(std::min<int64_t>(std::min<int64_t>(1 + l->max_delay, a.frame_len), a.n));
 
   const int nt = l->num_tx, nr = l->num_rx;
-  const int threads = std::min(kBatchMaxThreads, std::max(64, l->num_subcarriers / 2));
+  int threads = std::min(kBatchMaxThreads, std::max(64, l->num_subcarriers / 2));
+  if (l->workgroup_size != 0) {
+    const int w = l->workgroup_size;
+    if (w < 64 || w > kBatchMaxThreads || (w & (w - 1))) {
+      set_error("hpx_run_batch: workgroup_size must be 0 or a power of two in [64, %d]", kBatchMaxThreads);
+      return HPX_ERR_INVALID;
+    }
+    threads = w;
+  }
   const int pairs = std::max(1, l->num_subcarriers / (2 * threads));
-  if (l->num_subcarriers * std::max(nt, nr) > kFftMaxPerThread * threads) {
-    set_error("hpx_run_batch: %d streams of %d subcarriers exceed one workgroup's FFT capacity", std::max(nt, nr),
-              l->num_subcarriers);
+  if (pairs != 1 && pairs != 2 && pairs != 4) {
+    set_error("hpx_run_batch: %d subcarriers need %d sample pairs per thread at %d threads (1, 2 or 4)",
+              l->num_subcarriers, pairs, threads);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  if (l->num_subcarriers > kFftMaxPerThread * threads) {
+    set_error("hpx_run_batch: %d subcarriers exceed one workgroup's FFT capacity", l->num_subcarriers);
     return HPX_ERR_UNSUPPORTED;
   }
   if (nt * a.dx > kTailPerThread * threads) {
@@ -448,7 +466,7 @@ int make_plan(const hpx_link* l, Plan* out) {
   out->pairs = pairs;
 
   // LDS layout; the prologue tables (uniforms, SoS table) alias the signal buffer
-  const int rows = std::max(nr, nt);
+  const int rows = std::max(nr, nt) + 1;  // signal rows + the CSI row
   int64_t o = 0;
   a.off_x = 0;
   const int64_t xbytes = static_cast<int64_t>(rows) * a.xs * sizeof(float2);
@@ -496,12 +514,12 @@ int launch(K kernel, const Plan& plan, int64_t num_drops, hipStream_t stream) {
   return check_launch("hpx_run_batch");
 }
 
-template <int NT>
+template <int NT, int MAXT>
 int launch_pairs(const Plan& plan, int64_t num_drops, hipStream_t stream) {
   switch (plan.pairs) {
-    case 1: return launch(batch_kernel<NT, NT, 1>, plan, num_drops, stream);
-    case 2: return launch(batch_kernel<NT, NT, 2>, plan, num_drops, stream);
-    case 4: return launch(batch_kernel<NT, NT, 4>, plan, num_drops, stream);
+    case 1: return launch(batch_kernel<NT, NT, 1, MAXT>, plan, num_drops, stream);
+    case 2: return launch(batch_kernel<NT, NT, 2, MAXT>, plan, num_drops, stream);
+    case 4: return launch(batch_kernel<NT, NT, 4, MAXT>, plan, num_drops, stream);
   }
   set_error("hpx_run_batch: unsupported samples per thread");
   return HPX_ERR_UNSUPPORTED;
@@ -540,10 +558,10 @@ extern "C" int hpx_run_batch(const hpx_link* link_host, const int32_t* delays, c
   a.first_drop = first_drop;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   switch (link_host->num_tx) {
-    case 1: return launch_pairs<1>(plan, num_drops, s);
-    case 2: return launch_pairs<2>(plan, num_drops, s);
-    case 4: return launch_pairs<4>(plan, num_drops, s);
-    case 8: return launch_pairs<8>(plan, num_drops, s);
+    case 1: return plan.threads <= 256 ? launch_pairs<1, 256>(plan, num_drops, s) : launch_pairs<1, 512>(plan, num_drops, s);
+    case 2: return plan.threads <= 256 ? launch_pairs<2, 256>(plan, num_drops, s) : launch_pairs<2, 512>(plan, num_drops, s);
+    case 4: return plan.threads <= 256 ? launch_pairs<4, 256>(plan, num_drops, s) : launch_pairs<4, 512>(plan, num_drops, s);
+    case 8: return plan.threads <= 256 ? launch_pairs<8, 256>(plan, num_drops, s) : launch_pairs<8, 512>(plan, num_drops, s);
   }
   set_error("hpx_run_batch: unreachable antenna configuration");
   return HPX_ERR_UNSUPPORTED;

@@ -82,15 +82,20 @@ __device__ __forceinline__ void fft_pass(float2* data, int count, int stride, in
 
 // data: vector v at data + v * stride; tw: N entries (LDS or global).
 // Must be called by every thread of the workgroup; ends with a barrier.
-// Requires count * N <= kFftMaxPerThread * blockDim.x.
+// Vectors are transformed in groups of at most kFftMaxPerThread * blockDim.x / N.
 template <int SIGN>
 __device__ void lds_fft(float2* data, int count, int stride, int log2n, const float2* tw) {
-  int log2ns = 0;
-  while (log2n - log2ns >= 2) {
-    fft_pass<SIGN, 4>(data, count, stride, log2n, log2ns, tw);
-    log2ns += 2;
+  const int group = max(1, (kFftMaxPerThread * static_cast<int>(blockDim.x)) >> log2n);
+  for (int g0 = 0; g0 < count; g0 += group) {
+    const int cnt = min(group, count - g0);
+    float2* base = data + g0 * stride;
+    int log2ns = 0;
+    while (log2n - log2ns >= 2) {
+      fft_pass<SIGN, 4>(base, cnt, stride, log2n, log2ns, tw);
+      log2ns += 2;
+    }
+    if (log2n - log2ns == 1) fft_pass<SIGN, 2>(base, cnt, stride, log2n, log2ns, tw);
   }
-  if (log2n - log2ns == 1) fft_pass<SIGN, 2>(data, count, stride, log2n, log2ns, tw);
 }
 
 // Grid column q of an N-subcarrier OFDM symbol -> FFT bin (composition of the

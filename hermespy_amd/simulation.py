@@ -50,7 +50,7 @@ class BatchedLink:
             link.num_paths, link.num_sinusoids, link.max_delay, det, int(bool(link.dc_suppression)),
             float(link.bandwidth), float(link.doppler),
             float(link.doppler if link.los_doppler is None else link.los_doppler), float(link.gain),
-            float(link.device_power))
+            float(link.device_power), int(link.workgroup_size))
         self.lds_bytes = int(lib.hpx_run_batch_lds_bytes(ctypes.byref(self.c_link)))
         if self.lds_bytes <= 0:
             # surface the validation message

@@ -174,6 +174,7 @@ typedef struct hpx_link {
   double los_doppler;  /* LOS doppler (rad/s) */
   double gain;         /* linear channel gain */
   double device_power; /* transmit device power */
+  int32_t workgroup_size; /* threads per drop: 0 = library default, else a power of two in [64, 512] */
 } hpx_link;
 
 /* Device arrays of hpx_run_batch:
