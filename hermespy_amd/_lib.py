@@ -78,6 +78,8 @@ SIGNATURES = {
     "hpx_spatial_detector_host": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
     "hpx_stream_bits": (c_int, [c_u64, c_u32, c_u32, c_i64, c_vp, c_vp]),
     "hpx_realization_normals": (c_int, [c_u64, c_u32, c_i64, c_vp, c_vp]),
+    "hpx_ls_estimate": (c_int, [c_vp, c_int, c_i64, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_vp, c_vp,
+                                c_vp, c_vp]),
 }
 
 _lib: ctypes.CDLL | None = None

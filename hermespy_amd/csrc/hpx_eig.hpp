@@ -21,20 +21,27 @@ constexpr int kEigMax = 8;
 // unitary, columns are eigenvectors.  Returns the number of sweeps used.
 __host__ __device__ inline int hermitian_jacobi(int n, double* are, double* aim, double* vre, double* vim,
                                                 int max_sweeps = 30) {
+#pragma unroll 1
   for (int i = 0; i < n; ++i)
+#pragma unroll 1
     for (int j = 0; j < n; ++j) {
       vre[i * n + j] = (i == j) ? 1.0 : 0.0;
       vim[i * n + j] = 0.0;
     }
   int sweep = 0;
+#pragma unroll 1
   for (; sweep < max_sweeps; ++sweep) {
     double off = 0.0, diag = 0.0;
+#pragma unroll 1
     for (int i = 0; i < n; ++i) {
       diag += are[i * n + i] * are[i * n + i];
+#pragma unroll 1
       for (int j = i + 1; j < n; ++j) off += are[i * n + j] * are[i * n + j] + aim[i * n + j] * aim[i * n + j];
     }
     if (off <= 1e-30 * diag || off == 0.0) break;
+#pragma unroll 1
     for (int p = 0; p < n - 1; ++p)
+#pragma unroll 1
       for (int q = p + 1; q < n; ++q) {
         const double xr = are[p * n + q], xi = aim[p * n + q];
         const double r = sqrt(xr * xr + xi * xi);
@@ -46,6 +53,7 @@ __host__ __device__ inline int hermitian_jacobi(int n, double* are, double* aim,
         const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
         // G = diag(1, e^{-i phi}) [[c, s], [-s, c]] on columns p, q:
         //   col_p' = c col_p - s e^{-i phi} col_q,  col_q' = s col_p + c e^{-i phi} col_q
+#pragma unroll 1
         for (int k = 0; k < n; ++k) {
           const double pr = are[k * n + p], pi = aim[k * n + p];
           const double qr = are[k * n + q], qi = aim[k * n + q];
@@ -63,6 +71,7 @@ __host__ __device__ inline int hermitian_jacobi(int n, double* are, double* aim,
           vim[k * n + q] = s * vpi + c * vqei;
         }
         //   row_p' = c row_p - s e^{i phi} row_q,  row_q' = s row_p + c e^{i phi} row_q
+#pragma unroll 1
         for (int k = 0; k < n; ++k) {
           const double pr = are[p * n + k], pi = aim[p * n + k];
           const double qr = are[q * n + k], qi = aim[q * n + k];
@@ -92,9 +101,12 @@ __host__ __device__ inline void spatial_detector(int nr, int nt, const double* s
   double* gim = work + kEigMax * kEigMax;
   double* vre = work + 2 * kEigMax * kEigMax;
   double* vim = work + 3 * kEigMax * kEigMax;
+#pragma unroll 1
   for (int i = 0; i < nt; ++i)
+#pragma unroll 1
     for (int j = 0; j < nt; ++j) {
       double sr = 0.0, si = 0.0;
+#pragma unroll 1
       for (int r = 0; r < nr; ++r) {  // conj(S_ri) S_rj
         const double ar = s_ri[2 * (r * nt + i)], ai = s_ri[2 * (r * nt + i) + 1];
         const double br = s_ri[2 * (r * nt + j)], bi = s_ri[2 * (r * nt + j) + 1];
@@ -105,15 +117,20 @@ __host__ __device__ inline void spatial_detector(int nr, int nt, const double* s
       gim[i * nt + j] = si;
     }
   hermitian_jacobi(nt, gre, gim, vre, vim);
+#pragma unroll 1
   for (int i = 0; i < nt; ++i) lambda[i] = gre[i * nt + i];
+#pragma unroll 1
   for (int i = 0; i < nt * nt; ++i) {
     v_ri[2 * i] = vre[i];
     v_ri[2 * i + 1] = vim[i];
   }
   // A[i][r] = sum_k conj(V[k][i]) conj(S[r][k])
+#pragma unroll 1
   for (int i = 0; i < nt; ++i)
+#pragma unroll 1
     for (int r = 0; r < nr; ++r) {
       double ar = 0.0, ai = 0.0;
+#pragma unroll 1
       for (int k = 0; k < nt; ++k) {
         const double vr = vre[k * nt + i], vi = -vim[k * nt + i];
         const double sr = s_ri[2 * (r * nt + k)], si = -s_ri[2 * (r * nt + k) + 1];

@@ -84,7 +84,7 @@ __device__ __forceinline__ void fft_pass(float2* data, int count, int stride, in
 // Must be called by every thread of the workgroup; ends with a barrier.
 // Vectors are transformed in groups of at most kFftMaxPerThread * blockDim.x / N.
 template <int SIGN>
-__device__ void lds_fft(float2* data, int count, int stride, int log2n, const float2* tw) {
+__device__ __forceinline__ void lds_fft(float2* data, int count, int stride, int log2n, const float2* tw) {
   const int group = max(1, (kFftMaxPerThread * static_cast<int>(blockDim.x)) >> log2n);
   for (int g0 = 0; g0 < count; g0 += group) {
     const int cnt = min(group, count - g0);

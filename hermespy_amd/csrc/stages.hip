@@ -597,3 +597,70 @@ extern "C" int hpx_realization_normals(uint64_t seed, uint32_t drop, int64_t cou
                      HPX_STREAM(stream), seed, drop, count, normals);
   return check_launch("hpx_realization_normals");
 }
+
+// ------------------------------------------------------------------ pilot LS estimation
+__global__ void __launch_bounds__(256) ls_pilots_kernel(const float2* __restrict__ grid, int64_t ld, int npil,
+                                                        const int32_t* __restrict__ pidx,
+                                                        const float2* __restrict__ psym, float2* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (i >= npil) return;
+  const int64_t k = s * ld + pidx[i];
+  const float2 x = psym[i], y = grid[k];
+  const float inv = 1.0f / cabs2(x);
+  out[k] = cscale(cmulc(x, y), inv);  // y / x
+}
+
+__global__ void __launch_bounds__(256) ls_holes_kernel(int64_t ld, int nhole, const int32_t* __restrict__ hidx,
+                                                       const int32_t* __restrict__ pidx,
+                                                       const int32_t* __restrict__ verts,
+                                                       const float* __restrict__ w, float2* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (i >= nhole) return;
+  const int v0 = verts[3 * i];
+  if (v0 < 0) return;  // outside the pilots' hull: filled by the nearest element
+  float2* row = out + s * ld;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int k = 0; k < 3; ++k) {
+    const float2 h = row[pidx[verts[3 * i + k]]];
+    acc.x = fmaf(w[3 * i + k], h.x, acc.x);
+    acc.y = fmaf(w[3 * i + k], h.y, acc.y);
+  }
+  row[hidx[i]] = acc;
+}
+
+__global__ void __launch_bounds__(256) ls_fill_kernel(int64_t ld, int nfill, const int32_t* __restrict__ fidx,
+                                                      const int32_t* __restrict__ fsrc, float2* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (i >= nfill) return;
+  float2* row = out + s * ld;
+  row[fidx[i]] = row[fsrc[i]];
+}
+
+extern "C" int hpx_ls_estimate(const hpx_c64* grid, int num_streams, int64_t grid_size, int num_pilots,
+                               const int32_t* pilot_index, const hpx_c64* pilot_symbols, int num_holes,
+                               const int32_t* hole_index, const int32_t* hole_vertices, const float* hole_weights,
+                               int num_fill, const int32_t* fill_index, const int32_t* fill_source, hpx_c64* estimate,
+                               void* stream) {
+  HPX_REQUIRE(num_streams >= 1 && num_streams <= 65535 && grid_size >= 1 && num_pilots >= 1 && num_holes >= 0 &&
+                  num_fill >= 0 && static_cast<int64_t>(num_pilots) + num_holes <= grid_size,
+              "hpx_ls_estimate: invalid sizes");
+  HPX_REQUIRE(grid && pilot_index && pilot_symbols && estimate, "hpx_ls_estimate: null pointer");
+  HPX_REQUIRE((num_holes == 0 || (hole_index && hole_vertices && hole_weights)) &&
+                  (num_fill == 0 || (fill_index && fill_source)),
+              "hpx_ls_estimate: null pointer");
+  const hipStream_t s = HPX_STREAM(stream);
+  float2* out = reinterpret_cast<float2*>(estimate);
+  hipLaunchKernelGGL(ls_pilots_kernel, dim3((num_pilots + 255) / 256, num_streams), dim3(256), 0, s,
+                     reinterpret_cast<const float2*>(grid), grid_size, num_pilots, pilot_index,
+                     reinterpret_cast<const float2*>(pilot_symbols), out);
+  if (num_holes > 0)
+    hipLaunchKernelGGL(ls_holes_kernel, dim3((num_holes + 255) / 256, num_streams), dim3(256), 0, s, grid_size,
+                       num_holes, hole_index, pilot_index, hole_vertices, hole_weights, out);
+  if (num_fill > 0)
+    hipLaunchKernelGGL(ls_fill_kernel, dim3((num_fill + 255) / 256, num_streams), dim3(256), 0, s, grid_size,
+                       num_fill, fill_index, fill_source, out);
+  return check_launch("hpx_ls_estimate");
+}

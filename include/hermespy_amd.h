@@ -139,6 +139,19 @@ int hpx_ideal_csi(const double* sos, int num_paths, int num_terms, const int32_t
                   const hpx_c64* spatial, int num_rx, int num_tx, int words, int num_subcarriers, int prefix,
                   int dc_suppression, hpx_c64* csi, void* stream);
 
+/* Pilot least-squares channel estimation of num_streams grids of grid_size elements:
+ * estimate = grid / pilot at the pilots, the barycentric combination of three pilot
+ * estimates (hole_vertices index the pilot list, -1 = outside the pilots' hull) at the
+ * holes, then estimate[fill_index] = estimate[fill_source].  The plan (indices,
+ * weights) is the Delaunay-linear + nearest interpolation of scipy.interpolate.griddata
+ * for the frame's pilot pattern.  Replaces OrthogonalLeastSquaresChannelEstimation.
+ * estimate_channel, hermespy/modem/waveforms/orthogonal/channel.py:29-66 (per stream). */
+int hpx_ls_estimate(const hpx_c64* grid, int num_streams, int64_t grid_size, int num_pilots,
+                    const int32_t* pilot_index, const hpx_c64* pilot_symbols, int num_holes,
+                    const int32_t* hole_index, const int32_t* hole_vertices, const float* hole_weights,
+                    int num_fill, const int32_t* fill_index, const int32_t* fill_source, hpx_c64* estimate,
+                    void* stream);
+
 /* Per resource element equalisation: y [num_rx][K], h [num_rx][num_tx][K] -> x [num_tx][K].
  * HPX_DETECT_ZF: pseudo-inverse (num_rx == 1: y / sum_t h), replaces
  * ZeroForcingChannelEqualization.equalize_channel, hermespy/modem/waveform.py:235-250.

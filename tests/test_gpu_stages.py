@@ -202,3 +202,28 @@ def test_channel_mirror_realization_matches_reference_draw_order(cuda):
     os_ = fading.sample_from_normals(params, real.scalars, 2, 2, 1e6)
     np.testing.assert_array_equal(s.nlos_angles, os_.nlos_angles)
     np.testing.assert_array_equal(s.spatial_response, os_.spatial_response)
+
+
+def test_ls_estimate(cuda):
+    """Pilot LS + griddata interpolation (hpx_ls_estimate) against the oracle's scipy path, 2 streams."""
+
+    from hermespy_amd.estimation import OrthogonalLeastSquaresChannelEstimation
+    from hermespy_amd.modem import ElementType, GridElement, GridResource, OFDMWaveform, SymbolSection
+    from oracle import estimation as oest
+
+    n = 512
+    res = [GridResource(1, "CYCLIC", 0.0, [GridElement("DATA", n)]),
+           GridResource(n // 4, "CYCLIC", 0.0, [GridElement("REFERENCE", 1), GridElement("DATA", 3)])]
+    pattern = [0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0]
+    wf = OFDMWaveform(res, [SymbolSection(1, pattern)], n, modulation_order=16)
+    est = OrthogonalLeastSquaresChannelEstimation(wf, pilot_sequence=mapping.constellation(4))
+    rng = np.random.default_rng(7)
+    grid = ((rng.standard_normal((2, 14, n)) + 1j * rng.standard_normal((2, 14, n))) / np.sqrt(2)).astype(np.complex64)
+    tgrid = dev(grid)
+    got = est.estimate_channel(tgrid).cpu().numpy()
+    mask = wf.resource_mask()
+    frame = ofdm.Frame(n, np.zeros(14, int), np.where(mask[ElementType.REFERENCE.value], ofdm.REFERENCE,
+                                                      ofdm.DATA).astype(np.int8))
+    for s in range(2):
+        ref = oest.ls_estimate(frame, grid[s].astype(np.complex128), est.pilots)
+        assert rel_err(got[s], ref) < 1e-6
