@@ -251,30 +251,62 @@ batch_kernel(BatchArgs a) {
     for (int k = 0; k < PAIRS; ++k) {
       const int i = tid + k * nthreads;  // sample pair of the data window
       if (2 * i < n) {
+        const int nl = 2 * i;
+        const int64_t nabs = win0 + nl;
+        float2 z0[NT], z1[NT];
 #pragma unroll
-        for (int h2 = 0; h2 < 2; ++h2) {
-          const int nl = 2 * i + h2;
-          const int64_t nabs = win0 + nl;
-          float2 z[NT];
-#pragma unroll
-          for (int t = 0; t < NT; ++t) z[t] = make_float2(0.f, 0.f);
+        for (int t = 0; t < NT; ++t) z0[t] = z1[t] = make_float2(0.f, 0.f);
+        const float step = 2.0f / static_cast<float>(1 << a.log2b);  // offset of the next sample in a block
+        if (win0 >= a.maxd) {
+          // no delayed index reaches before the frame start: no per-lane checks
           for (int p = 0; p < L; ++p) {
             const int d = a.delays[p];
-            const int64_t m = nabs - d;
-            if (d < 0 || d > a.maxd || m < 0) continue;
-            const float2 h =
-                taylor_eval(coef + (p * a.nb + static_cast<int>(tb.block_of(m))) * kTaylorCoeffs, tb.offset(m));
+            if (d < 0 || d > a.maxd) continue;
+            const int m = static_cast<int>(nabs) - d;
+            const int b0 = m >> a.log2b;
+            const float2* c0 = coef + (p * a.nb + b0) * kTaylorCoeffs;
+            const float s0 = tb.offset(m);
+            const float2 h0 = taylor_eval(c0, s0);
+            // the pair's second sample opens the next block when m ends one
+            const bool wrap = ((m + 1) >> a.log2b) != b0;
+            const float2 h1 = wrap ? taylor_eval(c0 + kTaylorCoeffs, tb.offset(m + 1)) : taylor_eval(c0, s0 + step);
             const float2* xc = xbuf + x0 + nl - d;
 #pragma unroll
-            for (int t = 0; t < NT; ++t) z[t] = cfma(h, xc[t * a.xs], z[t]);
+            for (int t = 0; t < NT; ++t) {
+              z0[t] = cfma(h0, xc[t * a.xs], z0[t]);
+              z1[t] = cfma(h1, xc[t * a.xs + 1], z1[t]);
+            }
           }
+        } else {
+          for (int p = 0; p < L; ++p) {
+            const int d = a.delays[p];
+            if (d < 0 || d > a.maxd) continue;
+            const int64_t m = nabs - d;
+            const float2* xc = xbuf + x0 + nl - d;
+            if (m >= 0) {
+              const float2 h0 = taylor_eval(coef + (p * a.nb + static_cast<int>(tb.block_of(m))) * kTaylorCoeffs,
+                                            tb.offset(m));
 #pragma unroll
-          for (int r = 0; r < NR; ++r) {
-            float2 acc = make_float2(0.f, 0.f);
+              for (int t = 0; t < NT; ++t) z0[t] = cfma(h0, xc[t * a.xs], z0[t]);
+            }
+            if (m + 1 >= 0) {
+              const float2 h1 = taylor_eval(
+                  coef + (p * a.nb + static_cast<int>(tb.block_of(m + 1))) * kTaylorCoeffs, tb.offset(m + 1));
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc = cfma(smat[r * NT + t], z[t], acc);
-            y[k][h2][r] = acc;
+              for (int t = 0; t < NT; ++t) z1[t] = cfma(h1, xc[t * a.xs + 1], z1[t]);
+            }
           }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          float2 a0 = make_float2(0.f, 0.f), a1 = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            a0 = cfma(smat[r * NT + t], z0[t], a0);
+            a1 = cfma(smat[r * NT + t], z1[t], a1);
+          }
+          y[k][0][r] = a0;
+          y[k][1][r] = a1;
         }
         // noise sample n uses words (0,1) (n even) or (2,3) (n odd) of call n/2
 #pragma unroll

@@ -92,9 +92,10 @@ __device__ __forceinline__ double2 normal_pair_f64(uint4 w) {
 __device__ __forceinline__ float2 complex_normal_f32(uint32_t a, uint32_t b) {
   const float u1 = (static_cast<float>(a >> 8) + 0.5f) * 5.9604644775390625e-08f;  // 2^-24
   const float u2 = static_cast<float>(b >> 8) * 5.9604644775390625e-08f;
-  const float rho = sqrtf(-2.0f * logf(u1));
-  float s, c;
-  sincospif(2.0f * u2, &s, &c);
+  // hardware transcendentals: v_log_f32 is log2 (-2 ln u = -2 ln2 log2 u) and
+  // v_sin_f32 / v_cos_f32 take revolutions, i.e. exactly the angle 2 pi u2
+  const float rho = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+  const float s = __builtin_amdgcn_sinf(u2), c = __builtin_amdgcn_cosf(u2);
   return make_float2(rho * c, rho * s);
 }
 
