@@ -20,7 +20,12 @@ import time
 import numpy as np
 
 METRIC = "MC drops/s (4x4 1024-SC OFDM, MMSE) at BER parity; learned vs LMMSE est."
-FP32_VECTOR_PEAK_TFLOPS = 157.3
+FP32_VECTOR_PEAK_TFLOPS = 157.3  # published MI355X FP32 peak (packed FP32 math)
+FP32_UNPACKED_PEAK_TFLOPS = 78.6  # one FP32 FMA per lane and cycle: the kernels are built without packed FP32
+# HBM bytes of one fused 4x4 drop, from rocprofv3 FETCH_SIZE / WRITE_SIZE over 4096-drop launches
+# (FETCH x2.0 and WRITE x1.0, the factors a 256 MiB device copy reads in the same process):
+# (4876.7 x 2 + 71371.7) KiB / 4096; almost all of it is register-spill stores
+HBM_BYTES_PER_DROP = (4876.66 * 2.0 + 71371.69) * 1024.0 / 4096.0
 DROPS_PER_STEP = 4096
 SEED = 20261202
 
@@ -86,6 +91,110 @@ def plain_gpu_path(cfg, budget_s: float = 5.0) -> dict:
     return {"value": done / dt, "unit": "drops/s", "drops": done}
 
 
+def estimator_leg(drops: int = 4096, reps: int = 10) -> dict:
+    """Pilot-based estimation on the 2x2 512-subcarrier comb-pilot link: NMSE and BER of the LMMSE
+    estimator and of the learned estimator against the ideal CSI, and the throughput of each
+    (pilot LS + interpolation, drops/s, HIP events)."""
+
+    import torch
+
+    from hermespy_amd import _lib
+    from hermespy_amd.configs import mimo2x2_512_pilots
+    from hermespy_amd.learned import WEIGHTS_PATH, ConvNetEstimator
+    from hermespy_amd.lmmse import LMMSEChannelEstimation
+    from hermespy_amd.simulation import BatchedLink
+
+    cfg = mimo2x2_512_pilots()
+    dev = torch.device("cuda")
+    snr = np.asarray(cfg.snr_db)
+    sidx = np.arange(drops) % snr.size
+    npw = torch.as_tensor(cfg.noise_power(snr[sidx]), dtype=torch.float64, device=dev)
+    link = BatchedLink(cfg)
+    e = cfg.words * cfg.num_subcarriers
+    err_ideal = torch.empty(drops, dtype=torch.int32, device=dev)
+    grid = torch.empty((drops, cfg.num_rx, cfg.words, cfg.num_subcarriers), dtype=torch.complex64, device=dev)
+    csi = torch.empty((drops, cfg.words, cfg.num_subcarriers), dtype=torch.complex64, device=dev)
+    spatial = torch.empty((drops, cfg.num_rx, cfg.num_tx), dtype=torch.complex64, device=dev)
+    link.run(SEED, 0, npw, err_ideal, doppler=cfg.draw_doppler(0, drops), grid=grid, csi=csi, spatial=spatial)
+    lm = LMMSEChannelEstimation(cfg)
+    rows = lm.rows_by_set(sidx)
+    out = torch.empty((cfg.num_tx, drops, cfg.num_rx, e), dtype=torch.complex64, device=dev)
+    lib, st = _lib.load(), _lib.stream_handle()
+
+    def timed(fn) -> float:
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return drops * reps / (a.elapsed_time(b) / 1e3)
+
+    def quality(est) -> dict:
+        err = torch.zeros(drops, dtype=torch.float64, device=dev)
+        pw = torch.zeros(drops, dtype=torch.float64, device=dev)
+        _lib.check(lib.hpx_estimate_error(_lib.ptr(est), _lib.ptr(csi), _lib.ptr(spatial), drops, cfg.num_rx, cfg.num_tx,
+                                          e, _lib.ptr(err), _lib.ptr(pw), st))
+        bit_err = torch.empty(drops, dtype=torch.int32, device=dev)
+        _lib.check(lib.hpx_detect_estimated(_lib.ptr(grid), _lib.ptr(est), drops, cfg.num_rx, cfg.num_tx, cfg.words,
+                                            cfg.num_subcarriers, _lib.ptr(link.elements), _lib.ptr(link.word_data),
+                                            cfg.num_data, cfg.modulation_order, _lib.HPX_DETECT_MMSE, _lib.ptr(npw), SEED,
+                                            0, _lib.ptr(bit_err), None, st))
+        en, pn, bn = err.cpu().numpy(), pw.cpu().numpy(), bit_err.cpu().numpy()
+        nmse = [float(10 * np.log10(en[sidx == k].sum() / pn[sidx == k].sum())) for k in range(snr.size)]
+        ber = [float(bn[sidx == k].sum() / ((sidx == k).sum() * cfg.bits_per_drop)) for k in range(snr.size)]
+        return {"nmse_db": nmse, "ber": ber}
+
+    res = {"workload": cfg.name, "drops": drops, "snr_db": snr.tolist()}
+    ideal = err_ideal.cpu().numpy()
+    res["ideal_csi_ber"] = [float(ideal[sidx == k].sum() / ((sidx == k).sum() * cfg.bits_per_drop))
+                            for k in range(snr.size)]
+    ls = lm.pilot_ls(grid, drops)
+    res["lmmse"] = {"drops_per_s": timed(lambda: lm.apply(lm.pilot_ls(grid, drops), sidx, out, rows)),
+                    "dtype": "f32", **quality(lm.apply(ls, sidx, out, rows))}
+    if WEIGHTS_PATH.exists():
+        net = ConvNetEstimator.load(cfg)
+        res["learned"] = {"drops_per_s": timed(lambda: net.estimate(lm.pilot_ls(grid, drops), out)),
+                          "dtype": "bf16 weights and activations, f32 accumulate", "params": net.num_params,
+                          **quality(net.estimate(ls, out))}
+        res["throughput_ratio"] = res["learned"]["drops_per_s"] / res["lmmse"]["drops_per_s"]
+        gap = np.asarray(res["learned"]["nmse_db"]) - np.asarray(res["lmmse"]["nmse_db"])
+        res["nmse_gap_db"] = gap.tolist()
+        res["max_nmse_gap_db"] = float(gap.max())
+    else:
+        res["learned"] = None
+    return res
+
+
+def uneven_leg(drops: int = 8192, reps: int = 5) -> dict:
+    """The mixed population (antennas 1/2/4/8, N 128..2048, 5..23 paths, Doppler 10..900 Hz, QPSK to
+    64-QAM, 11 SNR points) as one graph-captured submission after per-class autotuning."""
+
+    import torch
+
+    from hermespy_amd.uneven import UnevenBatch, draw_population, tune
+
+    pop = draw_population(drops)
+    ub = UnevenBatch(pop)
+    tuning = tune(ub, reps=2)
+    ub.capture()
+    ub.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        ub.replay()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    bits = float(ub.bits.sum())
+    return {"workload": "uneven mix (8x8 at 2048 subcarriers excluded)", "drops": drops, "ms_per_launch": ms,
+            "drops_per_s": drops / (ms / 1e3), "bits_per_s": bits / (ms / 1e3), "classes": ub.num_classes,
+            "workgroup_choice": tuning["choice"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +205,8 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain single-drop GPU path")
     ap.add_argument("--no-tune", action="store_true", help="use the default workgroup size instead of the autotuner's")
+    ap.add_argument("--no-estimators", action="store_true", help="skip the pilot-estimation leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the heterogeneous-batch leg")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (counter calibration for profiles)")
     args = ap.parse_args()
@@ -170,13 +281,19 @@ def main() -> int:
         "wall_s_timed": wall,
         "ber_by_snr": [float(b) for b in ber],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "batch_kernel<4,4,P,T>", "flops_per_drop": algorithmic_flops_per_drop(cfg)},
+                     "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": HBM_BYTES_PER_DROP * n,
+                     "kernel": "batch_kernel<4,4,P,T>", "flops_per_drop": algorithmic_flops_per_drop(cfg),
+                     "peak_unpacked_fp32": FP32_UNPACKED_PEAK_TFLOPS,
+                     "frac_unpacked_fp32": achieved / FP32_UNPACKED_PEAK_TFLOPS},
     }
     if not args.no_plain:
         plain = plain_gpu_path(cfg)
         out["plain_gpu_path"] = plain
         out["speedup_vs_plain_gpu_path"] = drops_per_s / plain["value"]
+    if not args.no_estimators:
+        out["estimators"] = estimator_leg()
+    if not args.no_uneven:
+        out["uneven_batch"] = uneven_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
     print(json.dumps(out), flush=True)

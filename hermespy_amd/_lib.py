@@ -16,6 +16,8 @@ import torch  # noqa: F401  (must precede the library load, see module docstring
 
 LIB_PATH = Path(__file__).resolve().parent / "_native" / "libhermespy_amd.so"
 
+ABI_VERSION = 2
+
 HPX_OK = 0
 HPX_ERR_INVALID = -1
 HPX_ERR_UNSUPPORTED = -2
@@ -27,6 +29,8 @@ HPX_DETECT_MMSE = 1
 STREAM_REALIZATION = 0x000
 STREAM_BITS = 0x100
 STREAM_NOISE = 0x200
+
+ELEMENT_NULL = -2147483648
 
 c_int = ctypes.c_int
 c_i32 = ctypes.c_int32
@@ -54,6 +58,32 @@ class HpxLink(ctypes.Structure):
     ]
 
 
+class HpxFrame(ctypes.Structure):
+    """Mirror of ``hpx_frame`` (device pointers)."""
+
+    _fields_ = [("elements", c_vp), ("word_data", c_vp), ("pilots", c_vp), ("num_references", c_i32),
+                ("num_data", c_i32), ("max_word_data", c_i32)]
+
+
+class HpxBatchOut(ctypes.Structure):
+    """Mirror of ``hpx_batch_out`` (device pointers)."""
+
+    _fields_ = [("errors", c_vp), ("equalized", c_vp), ("grid", c_vp), ("csi", c_vp), ("spatial", c_vp)]
+
+
+class HpxDropTables(ctypes.Structure):
+    """Mirror of ``hpx_drop_tables`` (device pointers)."""
+
+    _fields_ = [("delays", c_vp), ("path_gains", c_vp), ("corr_rx", c_vp), ("corr_tx", c_vp)]
+
+
+class HpxUnevenClass(ctypes.Structure):
+    """Mirror of ``hpx_uneven_class``."""
+
+    _fields_ = [("first", c_i64), ("count", c_i64), ("num_tx", c_i32), ("threads", c_i32), ("pairs", c_i32),
+                ("reserved", c_i32), ("lds_bytes", c_i64)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "hpx_abi_version": (c_int, []),
@@ -74,10 +104,32 @@ SIGNATURES = {
     "hpx_count_errors": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
     "hpx_run_batch": (c_int, [ctypes.POINTER(HpxLink), c_vp, c_vp, c_vp, c_vp, c_u64, c_i64, c_i64, c_vp, c_vp, c_vp,
                               c_vp]),
+    "hpx_run_batch_frames": (c_int, [ctypes.POINTER(HpxLink), ctypes.POINTER(HpxFrame), c_vp, c_vp, c_vp, c_vp,
+                                     c_vp, c_u64, c_i64, c_i64, c_vp, ctypes.POINTER(HpxBatchOut), c_vp]),
+    "hpx_uneven_plan_bytes": (c_i64, []),
+    "hpx_uneven_prepare": (c_int, [ctypes.POINTER(HpxLink), ctypes.POINTER(HpxDropTables), c_vp, c_i64, c_u64, c_vp,
+                                   c_vp, c_vp, c_vp, ctypes.POINTER(HpxUnevenClass), c_int, ctypes.POINTER(c_int)]),
+    "hpx_uneven_run": (c_int, [c_vp, ctypes.POINTER(HpxUnevenClass), c_int, c_vp]),
     "hpx_run_batch_lds_bytes": (c_i64, [ctypes.POINTER(HpxLink)]),
     "hpx_spatial_detector_host": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
     "hpx_stream_bits": (c_int, [c_u64, c_u32, c_u32, c_i64, c_vp, c_vp]),
     "hpx_realization_normals": (c_int, [c_u64, c_u32, c_i64, c_vp, c_vp]),
+    "hpx_pilot_ls": (c_int, [c_vp, c_i64, c_int, c_int, c_i64, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "hpx_lmmse_apply": (c_int, [c_vp, c_int, c_i64, c_int, c_vp, c_i64, c_vp, c_int, c_vp, c_vp]),
+    "hpx_detect_estimated": (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_int, c_int,
+                                     c_vp, c_u64, c_i64, c_vp, c_vp, c_vp]),
+    "hpx_conv1d_forward": (c_int, [c_vp, c_i64, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
+    "hpx_conv1d_wgrad_workspace": (c_i64, [c_i64, c_int, c_int]),
+    "hpx_conv1d_wgrad": (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "hpx_gelu_backward": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "hpx_to_bf16": (c_int, [c_vp, c_i64, c_vp, c_vp]),
+    "hpx_net_input": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp]),
+    "hpx_net_output": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp]),
+    "hpx_estimate_error": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_i64, c_vp, c_vp, c_vp]),
+    "hpx_nmse_grad": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "hpx_adam_step": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float, c_i64, c_vp]),
+    "hpx_conv1d_weight_copies": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
     "hpx_ls_estimate": (c_int, [c_vp, c_int, c_i64, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_vp, c_vp,
                                 c_vp, c_vp]),
 }
@@ -99,7 +151,7 @@ def load() -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.hpx_abi_version() != 1:
+    if lib.hpx_abi_version() != ABI_VERSION:
         raise RuntimeError("libhermespy_amd ABI version mismatch")
     _lib = lib
     return lib

@@ -40,12 +40,23 @@ class LinkConfig:
     name: str = "link"
     snr_db: tuple = field(default=(0.0,))
     workgroup_size: int = 0  # threads per drop in the fused kernel (0: library default)
+    # frame with reference / null elements: (words, N) ElementType codes (REFERENCE 0, DATA 1, NULL 2)
+    frame_mask: np.ndarray | None = None
+    pilots: np.ndarray | None = None  # (num_tx, num_references) reference symbols of each stream
+    doppler_hz_range: tuple | None = None  # per-drop Doppler shift drawn uniformly from this range [Hz]
 
     def __post_init__(self) -> None:
         order = np.argsort(np.asarray(self.delays, dtype=np.float64))  # MultipathFadingChannel sorts, fading.py:707-711
         self.delays = np.asarray(self.delays, dtype=np.float64)[order]
         self.power_profile = np.asarray(self.power_profile, dtype=np.float64)[order]
         self.rice_factors = np.asarray(self.rice_factors, dtype=np.float64)[order]
+        if self.frame_mask is not None:
+            self.frame_mask = np.asarray(self.frame_mask, dtype=np.int8)
+            if self.frame_mask.shape != (self.words, self.num_subcarriers):
+                raise ValueError("frame_mask must have shape (words, num_subcarriers)")
+            nref = int(np.count_nonzero(self.frame_mask == 0))
+            if nref and (self.pilots is None or np.shape(self.pilots) != (self.num_tx, nref)):
+                raise ValueError("reference elements need (num_tx, num_references) pilot symbols")
 
     # ---------------------------------------------------------------- derived
     @property
@@ -61,8 +72,39 @@ class LinkConfig:
         return int(np.log2(self.modulation_order))
 
     @property
+    def num_data(self) -> int:
+        """Data symbols per stream and frame."""
+
+        if self.frame_mask is None:
+            return self.words * self.num_subcarriers
+        return int(np.count_nonzero(self.frame_mask == 1))
+
+    @property
+    def num_references(self) -> int:
+        return 0 if self.frame_mask is None else int(np.count_nonzero(self.frame_mask == 0))
+
+    @property
     def bits_per_drop(self) -> int:
-        return self.num_tx * self.words * self.num_subcarriers * self.bits_per_symbol
+        return self.num_tx * self.num_data * self.bits_per_symbol
+
+    def draw_doppler(self, first_drop: int, count: int) -> np.ndarray | None:
+        """Per-drop doppler_frequency (2 pi f_D, NLOS and LOS alike) of drops first_drop.., or None.
+
+        Drop i's shift is a function of i alone (a splitmix64 hash to a uniform), so any drop can
+        be re-simulated on its own."""
+
+        if self.doppler_hz_range is None:
+            return None
+        lo, hi = self.doppler_hz_range
+        i = np.arange(first_drop, first_drop + count, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = i * np.uint64(0x9E3779B97F4A7C15) + np.uint64(0x632BE59BD9B4E019)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        w = 2 * np.pi * (lo + (hi - lo) * u)
+        return np.stack([w, w], axis=-1)
 
     @property
     def num_paths(self) -> int:
@@ -136,4 +178,41 @@ def mimo4x4_1024() -> LinkConfig:
                       snr_db=tuple(float(s) for s in range(0, 31, 3)))
 
 
-CONFIGS = {"siso-64": siso_64, "mimo4x4-1024": mimo4x4_1024}
+def comb_pilot_frame(num_subcarriers: int, words: int, pilot_words, spacing: int, num_tx: int, seed: int = 20261202):
+    """Mask and per-stream pilots of a comb frame: ``GridResource`` [REF 1, DATA spacing-1] in the pilot
+    words, all-data words elsewhere.  Reference element j of a pilot word belongs to stream j % num_tx
+    (the others stay silent there), so every stream is estimated from its own pilots; the pilot values
+    are a fixed QPSK sequence (a ``CustomPilotSymbolSequence``)."""
+
+    mask = np.ones((words, num_subcarriers), dtype=np.int8)
+    for w in pilot_words:
+        mask[w, ::spacing] = 0
+    nref = int(np.count_nonzero(mask == 0))
+    rng = np.random.default_rng(seed)
+    qpsk = ((2 * rng.integers(0, 2, nref) - 1) + 1j * (2 * rng.integers(0, 2, nref) - 1)) / np.sqrt(2.0)
+    owner = np.tile(np.arange(num_subcarriers // spacing) % num_tx, len(pilot_words))
+    pilots = np.zeros((num_tx, nref), dtype=np.complex128)
+    for t in range(num_tx):
+        pilots[t, owner == t] = qpsk[owner == t]
+    return mask, pilots
+
+
+def mimo2x2_512_pilots() -> LinkConfig:
+    """2x2 correlated (rho=0.5) 23-tap TDL over 3 us (the 4x4 link's channel) at 15.36 MHz, 512 subcarriers,
+    CP 64 (N/8), 14 words with comb QPSK pilots on every 4th subcarrier of words 2 and 11, Doppler
+    uniform in 10..500 Hz per drop, 16-QAM, MMSE."""
+
+    fs = 15.36e6
+    tau = np.arange(23) * 3e-6 / 22
+    p = np.exp(-tau / 1e-6)
+    k = np.zeros(23)
+    k[0] = 1.0
+    r = exponential_correlation(2, 0.5)
+    mask, pilots = comb_pilot_frame(512, 14, (2, 11), 4, 2)
+    return LinkConfig(2, 2, 512, 14, 64 / 512, 16, fs, tau, p / p.sum(), k, 2 * np.pi * 500.0, corr_rx=r, corr_tx=r,
+                      detector="mmse", name="mimo2x2-512sc-16qam-comb-pilots",
+                      snr_db=tuple(float(s) for s in range(0, 31, 3)), frame_mask=mask, pilots=pilots,
+                      doppler_hz_range=(10.0, 500.0))
+
+
+CONFIGS = {"siso-64": siso_64, "mimo4x4-1024": mimo4x4_1024, "mimo2x2-512-pilots": mimo2x2_512_pilots}

@@ -18,6 +18,10 @@
 #include <math.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 #include "hpx_common.hpp"
 #include "hpx_detect.hpp"
@@ -42,7 +46,7 @@ struct BatchArgs {
   // link
   int n, log2n, cp, words, bps, order, detector, dc;
   int paths, sinusoids, terms, nnorm;
-  int log2b, nb, dx, maxd, keep_below, xs, bits_words;
+  int log2b, nb, dx, maxd, keep_below, xs, x0, cpl, bits_words;
   int64_t frame_len;
   double bandwidth, doppler, los_doppler, tx_scale;
   Modulation mod;
@@ -52,28 +56,35 @@ struct BatchArgs {
   const double* corr_rx;  // interleaved complex, may be null
   const double* corr_tx;
   const double* noise_power;
+  const double* doppler_drop;  // [drop][2] NLOS, LOS Doppler, or null (the link's)
   int32_t* errors;
   float2* equalized;
+  // frame (null elements: every element is data)
+  const int32_t* elements;
+  const int32_t* word_data;
+  const float2* pilots;
+  int nref, data_total;
+  // exports, or null
+  float2* out_grid;
+  float2* out_csi;
+  float2* out_spatial;
   uint64_t seed;
   int64_t first_drop;
   // LDS byte offsets
   int off_x, off_coef, off_abar, off_tw, off_bits, off_cons, off_s, off_det, off_red, off_tmp_sos, off_tmp_s64;
 };
 
-// bps bits of symbol q from an LSB-first word window (bit i of the result = i-th bit)
-__device__ __forceinline__ uint32_t symbol_bits(const uint32_t* bw, int q, int bps) {
-  const int bit = q * bps;
+// bps bits starting at bit `bit` of an LSB-first word window (bit i of the result = i-th bit)
+__device__ __forceinline__ uint32_t symbol_bits(const uint32_t* bw, int bit, int bps) {
   uint64_t v = bw[bit >> 5];
   if ((bit & 31) + bps > 32) v |= static_cast<uint64_t>(bw[(bit >> 5) + 1]) << 32;
   return static_cast<uint32_t>(v >> (bit & 31)) & ((1u << bps) - 1u);
 }
 
-// 512-thread workgroups target 4 waves per SIMD (128 VGPRs), i.e. two drops per CU,
-// which the LDS footprint of the 4x4 link allows; measured 1.6x faster than letting
-// the kernel take 256 VGPRs at one drop per CU.
-template <int NT, int NR, int PAIRS, int MAXT>
-__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(MAXT >= 512 ? 4 : 1)))
-batch_kernel(BatchArgs a) {
+// One drop, simulated by the whole workgroup.  local_drop indexes the per-drop arrays
+// (noise power, Doppler, errors, exports) and offsets the Philox drop index first_drop.
+template <int NT, int NR, int PAIRS>
+__device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t local_drop) {
   extern __shared__ __align__(16) unsigned char lds[];
   float2* xbuf = reinterpret_cast<float2*>(lds + a.off_x);
   float2* coef = reinterpret_cast<float2*>(lds + a.off_coef);
@@ -84,7 +95,7 @@ batch_kernel(BatchArgs a) {
   float2* smat = reinterpret_cast<float2*>(lds + a.off_s);
   int* red = reinterpret_cast<int*>(lds + a.off_red);
   constexpr int kRows = NT > NR ? NT : NR;
-  float2* csirow = xbuf + kRows * a.xs + a.dx + a.cp;              // [N] per-word CSI over the taps
+  float2* csirow = xbuf + kRows * a.xs + a.x0;                     // [N] per-word CSI over the taps
   double* uni = reinterpret_cast<double*>(lds + a.off_x);        // prologue only
   double* sos = reinterpret_cast<double*>(lds + a.off_tmp_sos);  // prologue only
   double* s64 = reinterpret_cast<double*>(lds + a.off_tmp_s64);  // prologue only: S in float64
@@ -94,7 +105,6 @@ batch_kernel(BatchArgs a) {
 
   const int tid = threadIdx.x;
   const int nthreads = blockDim.x;
-  const int64_t local_drop = blockIdx.x;
   const uint32_t drop = static_cast<uint32_t>(a.first_drop + local_drop);
   const int n = a.n;
   const int L = a.paths;
@@ -128,6 +138,7 @@ batch_kernel(BatchArgs a) {
       sincospi(2.0 * uni[10 * r + t], &si, &sr);
     }
     smat[e] = make_float2(static_cast<float>(sr), static_cast<float>(si));
+    if (a.out_spatial) a.out_spatial[local_drop * NR * NT + e] = smat[e];
     s64[2 * e] = sr;
     s64[2 * e + 1] = si;
   }
@@ -136,17 +147,19 @@ batch_kernel(BatchArgs a) {
     const int ns = a.sinusoids;
     const int o_los_ang = 100, o_nlos_ang = 100 + L, o_los_ph = 100 + L + L * ns, o_nlos_ph = 100 + 2 * L + L * ns;
     const double inv_fs = 1.0 / a.bandwidth;
+    const double dop = a.doppler_drop ? a.doppler_drop[2 * local_drop] : a.doppler;
+    const double los_dop = a.doppler_drop ? a.doppler_drop[2 * local_drop + 1] : a.los_doppler;
     for (int e = tid; e < L * a.terms; e += nthreads) {
       const int p = e / a.terms, k = e - p * a.terms;
       double* row = sos + 3 * e;
       if (k < ns) {
         const double ang = -M_PI + 2.0 * M_PI * uni[o_nlos_ang + p * ns + k];
-        row[0] = a.doppler * cos((2.0 * M_PI * (k + 1) + ang) / ns) * inv_fs;
+        row[0] = dop * cos((2.0 * M_PI * (k + 1) + ang) / ns) * inv_fs;
         row[1] = -M_PI + 2.0 * M_PI * uni[o_nlos_ph + p * ns + k];
         row[2] = a.path_gains[L + p];
       } else {
         const double ang = 2.0 * M_PI * uni[o_los_ang + p];
-        row[0] = a.los_doppler * cos(ang) * inv_fs;
+        row[0] = los_dop * cos(ang) * inv_fs;
         row[1] = -M_PI + 2.0 * M_PI * uni[o_los_ph + p];
         row[2] = a.path_gains[p];
       }
@@ -199,16 +212,21 @@ batch_kernel(BatchArgs a) {
   const float s2 = a.detector == HPX_DETECT_MMSE ? static_cast<float>(np) : 0.0f;
   const float rx_scale = static_cast<float>(1.0 / sqrt(static_cast<double>(n)));
   const int period = n + a.cp;
-  const int x0 = a.dx + a.cp;  // LDS column of the first data sample
+  const int x0 = a.x0;  // LDS column of the first data sample
   int errors = 0;
 
   // ---------------------------------------------------------------- words
   for (int w = 0; w < a.words; ++w) {
-    // bits of this word for every stream (stream HPX_STREAM_BITS + t)
-    {
-      const int nw = a.bits_words;
-      const int wb0 = w * nw;
-      const int c0 = wb0 >> 2, c1 = (wb0 + nw - 1) >> 2;
+    // bits of this word's data symbols for every stream (stream HPX_STREAM_BITS + t):
+    // LSB-first words fw .. fw + nw - 1 of the stream; the word's first bit sits at bit bo
+    const int wd0 = a.word_data ? a.word_data[w] : w * n;
+    const int wd1 = a.word_data ? a.word_data[w + 1] : (w + 1) * n;
+    const int64_t bit0 = static_cast<int64_t>(wd0) * a.bps;
+    const int bo = static_cast<int>(bit0 & 31);
+    if (wd1 > wd0) {
+      const int fw = static_cast<int>(bit0 >> 5);
+      const int lw = static_cast<int>((static_cast<int64_t>(wd1) * a.bps - 1) >> 5);
+      const int c0 = fw >> 2, c1 = lw >> 2;
       const int ncalls = c1 - c0 + 1;
       for (int e = tid; e < NT * ncalls; e += nthreads) {
         const int t = e / ncalls, c = c0 + (e - t * ncalls);
@@ -216,8 +234,8 @@ batch_kernel(BatchArgs a) {
         const uint32_t comp[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const int wi = 4 * c + k - wb0;
-          if (wi >= 0 && wi < nw) bits[t * nw + wi] = comp[k];
+          const int wi = 4 * c + k - fw;
+          if (wi >= 0 && wi <= lw - fw) bits[t * a.bits_words + wi] = comp[k];
         }
       }
     }
@@ -225,16 +243,28 @@ batch_kernel(BatchArgs a) {
     // map + place at the IFFT bins; clear the CSI row (row kRows: tap means at their delays)
     for (int e = tid; e < NT * n; e += nthreads) {
       const int t = e / n, q = e - t * n;
-      xbuf[t * a.xs + x0 + subcarrier_bin(q, n, a.dc)] = cons[symbol_bits(bits + t * a.bits_words, q, a.bps)];
+      const int el = a.elements ? a.elements[w * n + q] : q;
+      float2 v;
+      if (el >= 0) {
+        v = cons[symbol_bits(bits + t * a.bits_words, bo + el * a.bps, a.bps)];
+      } else if (el == HPX_ELEMENT_NULL) {
+        v = make_float2(0.f, 0.f);
+      } else {
+        const float2 pv = a.pilots[t * a.nref + (-1 - el)];
+        v = make_float2(static_cast<float>(static_cast<double>(pv.x) * a.tx_scale),
+                        static_cast<float>(static_cast<double>(pv.y) * a.tx_scale));
+      }
+      xbuf[t * a.xs + x0 + subcarrier_bin(q, n, a.dc)] = v;
     }
     for (int q = tid; q < n; q += nthreads) csirow[q] = make_float2(0.f, 0.f);
     __syncthreads();
     lds_fft<+1>(xbuf + x0, NT, a.xs, a.log2n, tw);
     // cyclic prefix; word means of the taps at their delays (paths sharing a delay add up,
     // taps beyond min(num_taps, N) are cut as fft(n=N) cuts them)
-    for (int e = tid; e < NT * a.cp; e += nthreads) {
-      const int t = e / a.cp, i = e - t * a.cp;
-      xbuf[t * a.xs + a.dx + i] = xbuf[t * a.xs + a.dx + n + i];
+    // (only the last cpl = min(CP, max delay) prefix samples are ever read by the delayed paths)
+    for (int e = tid; e < NT * a.cpl; e += nthreads) {
+      const int t = e / a.cpl, i = e - t * a.cpl;
+      xbuf[t * a.xs + x0 - a.cpl + i] = xbuf[t * a.xs + x0 + n - a.cpl + i];
     }
     if (tid == 0) {
       for (int p = 0; p < L; ++p) {
@@ -332,7 +362,7 @@ batch_kernel(BatchArgs a) {
 #pragma unroll
     for (int k = 0; k < kTailPerThread; ++k) {
       const int e = tid + k * nthreads;
-      if (e < NT * a.dx) tail[k] = xbuf[(e / a.dx) * a.xs + period + (e % a.dx)];
+      if (e < NT * a.dx) tail[k] = xbuf[(e / a.dx) * a.xs + x0 + n - a.dx + (e % a.dx)];
     }
     __syncthreads();
 #pragma unroll
@@ -371,6 +401,14 @@ batch_kernel(BatchArgs a) {
       float2 yv[NR];
 #pragma unroll
       for (int r = 0; r < NR; ++r) yv[r] = ycol[r * a.xs];
+      if (a.out_grid) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          a.out_grid[((local_drop * NR + r) * a.words + w) * static_cast<int64_t>(n) + q] = cscale(yv[r], rx_scale);
+      }
+      if (a.out_csi) a.out_csi[(local_drop * a.words + w) * static_cast<int64_t>(n) + q] = c;
+      const int el = a.elements ? a.elements[w * n + q] : q;
+      if (el < 0) continue;  // reference and null elements carry no bits
       const float c2 = cabs2(c);
       float2 u[NT];
 #pragma unroll
@@ -394,11 +432,9 @@ batch_kernel(BatchArgs a) {
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const uint32_t rx = demap_symbol(a.mod, static_cast<double>(xh[t].x), static_cast<double>(xh[t].y));
-        const uint32_t tx = symbol_bits(bits + t * a.bits_words, q, a.bps);
+        const uint32_t tx = symbol_bits(bits + t * a.bits_words, bo + el * a.bps, a.bps);
         errors += __popc(tx ^ rx);
-        if (a.equalized)
-          a.equalized[(local_drop * NT + t) * static_cast<int64_t>(a.words) * n + static_cast<int64_t>(w) * n + q] =
-              xh[t];
+        if (a.equalized) a.equalized[(local_drop * NT + t) * static_cast<int64_t>(a.data_total) + wd0 + el] = xh[t];
       }
     }
     __syncthreads();
@@ -411,6 +447,28 @@ batch_kernel(BatchArgs a) {
   if (tid == 0) a.errors[local_drop] = red[0];
 }
 
+// 512-thread workgroups of up to 4 antennas target 4 waves per SIMD (128 VGPRs), i.e. two
+// drops per CU, which the LDS footprint of the 4x4 link allows; measured 1.6x faster than
+// letting the kernel take 256 VGPRs at one drop per CU.  8x8 drops fill a CU's LDS alone.
+template <int NT, int MAXT>
+constexpr int batch_waves_per_eu() {
+  return MAXT >= 512 ? (NT <= 4 ? 4 : 2) : 1;
+}
+
+template <int NT, int NR, int PAIRS, int MAXT>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu<NT, MAXT>())))
+batch_kernel(BatchArgs a) {
+  drop_body<NT, NR, PAIRS>(a, blockIdx.x);
+}
+
+// Heterogeneous batches: workgroup i simulates the drop plans[i] describes; its first_drop is
+// the drop's Philox index and its per-drop pointers already point at the drop's entries.
+template <int NT, int NR, int PAIRS, int MAXT>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu<NT, MAXT>())))
+batch_kernel_multi(const BatchArgs* __restrict__ plans) {
+  drop_body<NT, NR, PAIRS>(plans[blockIdx.x], 0);
+}
+
 int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
 
 struct Plan {
@@ -419,9 +477,16 @@ struct Plan {
   BatchArgs args{};
 };
 
-// Validates the link and computes the launch geometry and LDS layout.
-int make_plan(const hpx_link* l, Plan* out) {
+// Validates the link (and frame, if any) and computes the launch geometry and LDS layout.
+int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out) {
   HPX_REQUIRE(l, "hpx_run_batch: null link");
+  if (f) {
+    HPX_REQUIRE(f->elements && f->word_data, "hpx_run_batch_frames: null frame arrays");
+    HPX_REQUIRE(f->num_references >= 0 && (f->num_references == 0 || f->pilots),
+                "hpx_run_batch_frames: reference elements need pilot symbols");
+    HPX_REQUIRE(f->num_data >= 0 && f->max_word_data >= 0 && f->max_word_data <= l->num_subcarriers,
+                "hpx_run_batch_frames: invalid data element counts");
+  }
   const int log2n = ilog2_exact(l->num_subcarriers);
   if (log2n < 5 || l->num_subcarriers > 4096) {
     set_error("hpx_run_batch: num_subcarriers must be a power of two in [32, 4096] (got %d)", l->num_subcarriers);
@@ -470,14 +535,21 @@ int make_plan(const hpx_link* l, Plan* out) {
     return HPX_ERR_UNSUPPORTED;
   }
   a.nb = static_cast<int>((a.frame_len + (int64_t(1) << a.log2b) - 1) >> a.log2b);
-  a.bits_words = (l->num_subcarriers * mod.bps) / 32;
+  // one word's data bits start anywhere in an LSB-first 32-bit word
+  const int max_word_data = f ? f->max_word_data : l->num_subcarriers;
+  a.bits_words = (max_word_data * mod.bps + 63) >> 5;
+  a.data_total = f ? f->num_data : l->words * l->num_subcarriers;
+  a.nref = f ? f->num_references : 0;
   a.maxd = l->max_delay;
   a.dx = std::max(0, l->max_delay - a.cp);
   if (a.dx > a.n) {
     set_error("hpx_run_batch: delay spread of %d samples exceeds one OFDM word", l->max_delay);
     return HPX_ERR_UNSUPPORTED;
   }
-  a.xs = a.dx + a.cp + a.n;
+  // LDS row: [previous word's last dx samples | last min(CP, D) prefix samples | N data samples]
+  a.cpl = std::min(a.cp, l->max_delay);
+  a.x0 = a.dx + a.cpl;  // == max delay
+  a.xs = a.x0 + a.n;
   a.xs += a.xs & 1;
   a.keep_below = static_cast<int>(std::min<int64_t>(std::min<int64_t>(1 + l->max_delay, a.frame_len), a.n));
 
@@ -528,7 +600,7 @@ int make_plan(const hpx_link* l, Plan* out) {
   a.off_tw = static_cast<int>(o);
   o += align16(static_cast<int64_t>(a.n) * sizeof(float2));
   a.off_bits = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(nt) * (a.bits_words + 1) * sizeof(uint32_t));
+  o += align16(static_cast<int64_t>(nt) * a.bits_words * sizeof(uint32_t));
   a.off_cons = static_cast<int>(o);
   o += align16(static_cast<int64_t>(a.order) * sizeof(float2));
   a.off_s = static_cast<int>(o);
@@ -545,11 +617,24 @@ int make_plan(const hpx_link* l, Plan* out) {
   return HPX_OK;
 }
 
+// Raises a kernel's dynamic LDS limit once per process and size (so that launches recorded
+// into a graph after a first eager run make no attribute calls).
+bool ensure_lds(const void* fn, int64_t bytes) {
+  static std::mutex mu;
+  static std::map<const void*, int64_t> granted;
+  if (bytes <= 64 * 1024) return true;
+  std::lock_guard<std::mutex> lock(mu);
+  int64_t& g = granted[fn];
+  if (g >= bytes) return true;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)) != hipSuccess)
+    return false;
+  g = bytes;
+  return true;
+}
+
 template <class K>
 int launch(K kernel, const Plan& plan, int64_t num_drops, hipStream_t stream) {
-  if (plan.lds > 64 * 1024 &&
-      hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          static_cast<int>(plan.lds)) != hipSuccess) {
+  if (!ensure_lds(reinterpret_cast<const void*>(kernel), plan.lds)) {
     set_error("hpx_run_batch: %lld bytes of LDS per workgroup not available", static_cast<long long>(plan.lds));
     return HPX_ERR_UNSUPPORTED;
   }
@@ -568,11 +653,139 @@ int launch_pairs(const Plan& plan, int64_t num_drops, hipStream_t stream) {
   return HPX_ERR_UNSUPPORTED;
 }
 
+template <int NT, int MAXT>
+int launch_multi(const BatchArgs* plans, const hpx_uneven_class& c, hipStream_t stream) {
+  const void* fn = nullptr;
+  switch (c.pairs) {
+    case 1: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 1, MAXT>); break;
+    case 2: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 2, MAXT>); break;
+    case 4: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 4, MAXT>); break;
+    default: set_error("hpx_uneven_run: unsupported samples per thread"); return HPX_ERR_UNSUPPORTED;
+  }
+  if (!ensure_lds(fn, c.lds_bytes)) {
+    set_error("hpx_uneven_run: %lld bytes of LDS per workgroup not available", static_cast<long long>(c.lds_bytes));
+    return HPX_ERR_UNSUPPORTED;
+  }
+  const BatchArgs* first = plans + c.first;
+  switch (c.pairs) {
+    case 1: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 1, MAXT>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
+    case 2: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 2, MAXT>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
+    default: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 4, MAXT>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
+  }
+  return check_launch("hpx_uneven_run");
+}
+
 }  // namespace
+
+extern "C" int64_t hpx_uneven_plan_bytes(void) { return static_cast<int64_t>(sizeof(BatchArgs)); }
+
+extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_host,
+                                  const uint32_t* drop_index_host, int64_t num_drops, uint64_t seed,
+                                  const double* noise_power, int32_t* errors, void* plans_host, int64_t* order_host,
+                                  hpx_uneven_class* classes_host, int max_classes, int* num_classes_host) {
+  HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff && max_classes >= 1, "hpx_uneven_prepare: invalid sizes");
+  HPX_REQUIRE(num_classes_host, "hpx_uneven_prepare: null pointer");
+  *num_classes_host = 0;
+  if (num_drops == 0) return HPX_OK;
+  HPX_REQUIRE(links_host && tables_host && drop_index_host && noise_power && errors && plans_host && order_host &&
+                  classes_host,
+              "hpx_uneven_prepare: null pointer");
+  std::vector<Plan> plans(static_cast<size_t>(num_drops));
+  for (int64_t i = 0; i < num_drops; ++i) {
+    const hpx_drop_tables& tb = tables_host[i];
+    HPX_REQUIRE(tb.delays && tb.path_gains, "hpx_uneven_prepare: drop %lld has no path tables",
+                static_cast<long long>(i));
+    HPX_REQUIRE((tb.corr_rx == nullptr) == (tb.corr_tx == nullptr),
+                "hpx_uneven_prepare: give both correlation matrices or none");
+    const int st = make_plan(&links_host[i], nullptr, &plans[i]);
+    if (st != HPX_OK) return st;
+    BatchArgs& a = plans[i].args;
+    a.delays = tb.delays;
+    a.path_gains = tb.path_gains;
+    a.corr_rx = reinterpret_cast<const double*>(tb.corr_rx);
+    a.corr_tx = reinterpret_cast<const double*>(tb.corr_tx);
+    a.noise_power = noise_power + i;
+    a.errors = errors + i;
+    a.seed = seed;
+    a.first_drop = drop_index_host[i];
+  }
+  // kernel class = (antennas, threads, samples per thread); heavier classes and drops first
+  auto key = [&](int64_t i) {
+    return std::make_tuple(links_host[i].num_tx, plans[i].threads, plans[i].pairs);
+  };
+  auto cost = [&](int64_t i) {
+    const hpx_link& l = links_host[i];
+    return static_cast<double>(l.num_tx) * l.num_rx * l.words * (l.num_subcarriers + l.prefix) * l.num_paths;
+  };
+  std::vector<int64_t> idx(static_cast<size_t>(num_drops));
+  for (int64_t i = 0; i < num_drops; ++i) idx[i] = i;
+  std::map<std::tuple<int, int, int>, double> class_cost;
+  for (int64_t i = 0; i < num_drops; ++i) class_cost[key(i)] = std::max(class_cost[key(i)], cost(i));
+  std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
+    const auto kx = key(x), ky = key(y);
+    if (kx != ky) {
+      const double cx = class_cost[kx], cy = class_cost[ky];
+      return cx != cy ? cx > cy : kx < ky;
+    }
+    return cost(x) > cost(y);
+  });
+  BatchArgs* out = static_cast<BatchArgs*>(plans_host);
+  int nc = 0;
+  for (int64_t k = 0; k < num_drops; ++k) {
+    const int64_t i = idx[k];
+    out[k] = plans[i].args;
+    order_host[k] = i;
+    const bool fresh = k == 0 || key(idx[k - 1]) != key(i);
+    if (fresh) {
+      if (nc == max_classes) {
+        set_error("hpx_uneven_prepare: more than %d kernel classes", max_classes);
+        return HPX_ERR_INVALID;
+      }
+      hpx_uneven_class& c = classes_host[nc++];
+      c.first = k;
+      c.count = 0;
+      c.num_tx = links_host[i].num_tx;
+      c.threads = plans[i].threads;
+      c.pairs = plans[i].pairs;
+      c.reserved = 0;
+      c.lds_bytes = 0;
+    }
+    hpx_uneven_class& c = classes_host[nc - 1];
+    c.count += 1;
+    c.lds_bytes = std::max<int64_t>(c.lds_bytes, plans[i].lds);
+  }
+  *num_classes_host = nc;
+  return HPX_OK;
+}
+
+extern "C" int hpx_uneven_run(const void* plans, const hpx_uneven_class* classes_host, int num_classes,
+                              void* stream) {
+  HPX_REQUIRE(num_classes >= 0, "hpx_uneven_run: invalid class count");
+  if (num_classes == 0) return HPX_OK;
+  HPX_REQUIRE(plans && classes_host, "hpx_uneven_run: null pointer");
+  const BatchArgs* p = static_cast<const BatchArgs*>(plans);
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int k = 0; k < num_classes; ++k) {
+    const hpx_uneven_class& c = classes_host[k];
+    HPX_REQUIRE(c.count >= 0 && c.count <= 0x7fffffff && c.first >= 0, "hpx_uneven_run: invalid class");
+    if (c.count == 0) continue;
+    int st = HPX_ERR_UNSUPPORTED;
+    const bool big = c.threads > 256;
+    switch (c.num_tx) {
+      case 1: st = big ? launch_multi<1, 512>(p, c, s) : launch_multi<1, 256>(p, c, s); break;
+      case 2: st = big ? launch_multi<2, 512>(p, c, s) : launch_multi<2, 256>(p, c, s); break;
+      case 4: st = big ? launch_multi<4, 512>(p, c, s) : launch_multi<4, 256>(p, c, s); break;
+      case 8: st = big ? launch_multi<8, 512>(p, c, s) : launch_multi<8, 256>(p, c, s); break;
+      default: set_error("hpx_uneven_run: antenna count %d not instantiated", c.num_tx);
+    }
+    if (st != HPX_OK) return st;
+  }
+  return HPX_OK;
+}
 
 extern "C" int64_t hpx_run_batch_lds_bytes(const hpx_link* link_host) {
   Plan plan;
-  if (make_plan(link_host, &plan) != HPX_OK) return 0;
+  if (make_plan(link_host, nullptr, &plan) != HPX_OK) return 0;
   return plan.lds;
 }
 
@@ -580,13 +793,26 @@ extern "C" int hpx_run_batch(const hpx_link* link_host, const int32_t* delays, c
                              const hpx_c128* corr_rx, const hpx_c128* corr_tx, uint64_t seed, int64_t first_drop,
                              int64_t num_drops, const double* noise_power, int32_t* errors, hpx_c64* equalized,
                              void* stream) {
+  hpx_batch_out out{};
+  out.errors = errors;
+  out.equalized = equalized;
+  return hpx_run_batch_frames(link_host, nullptr, delays, path_gains, corr_rx, corr_tx, nullptr, seed, first_drop,
+                              num_drops, noise_power, &out, stream);
+}
+
+extern "C" int hpx_run_batch_frames(const hpx_link* link_host, const hpx_frame* frame_host, const int32_t* delays,
+                                    const double* path_gains, const hpx_c128* corr_rx, const hpx_c128* corr_tx,
+                                    const double* doppler, uint64_t seed, int64_t first_drop, int64_t num_drops,
+                                    const double* noise_power, const hpx_batch_out* out_host, void* stream) {
+  HPX_REQUIRE(out_host, "hpx_run_batch: null output descriptor");
+  int32_t* errors = out_host->errors;
   HPX_REQUIRE(delays && path_gains && noise_power && errors, "hpx_run_batch: null pointer");
   HPX_REQUIRE((corr_rx == nullptr) == (corr_tx == nullptr), "hpx_run_batch: give both correlation matrices or none");
   HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff, "hpx_run_batch: invalid drop count");
   HPX_REQUIRE(first_drop >= 0 && first_drop + num_drops <= (int64_t(1) << 32),
               "hpx_run_batch: drop index exceeds 32 bits");
   Plan plan;
-  const int st = make_plan(link_host, &plan);
+  const int st = make_plan(link_host, frame_host, &plan);
   if (st != HPX_OK) return st;
   if (num_drops == 0) return HPX_OK;
   BatchArgs& a = plan.args;
@@ -596,7 +822,14 @@ extern "C" int hpx_run_batch(const hpx_link* link_host, const int32_t* delays, c
   a.corr_tx = reinterpret_cast<const double*>(corr_tx);
   a.noise_power = noise_power;
   a.errors = errors;
-  a.equalized = reinterpret_cast<float2*>(equalized);
+  a.equalized = reinterpret_cast<float2*>(out_host->equalized);
+  a.doppler_drop = doppler;
+  a.elements = frame_host ? frame_host->elements : nullptr;
+  a.word_data = frame_host ? frame_host->word_data : nullptr;
+  a.pilots = frame_host ? reinterpret_cast<const float2*>(frame_host->pilots) : nullptr;
+  a.out_grid = reinterpret_cast<float2*>(out_host->grid);
+  a.out_csi = reinterpret_cast<float2*>(out_host->csi);
+  a.out_spatial = reinterpret_cast<float2*>(out_host->spatial);
   a.seed = seed;
   a.first_drop = first_drop;
   const hipStream_t s = static_cast<hipStream_t>(stream);

@@ -1,0 +1,494 @@
+// Learned pilot-to-grid channel estimator: a 1-D convolutional network over the
+// pilot positions of one (receive, transmit) stream pair, bfloat16 weights and
+// activations on the MFMA units with float32 accumulation, and the kernels that
+// train it on drops the simulator generates (backward-data on the same MFMA
+// convolution, weight gradients, NMSE loss, Adam).
+//
+// Layout: sample s = (t * num_drops + d) * num_rx + r, positions p = 0..63,
+// channels innermost: activations [S][64][C].  Weights [Cout][5][Cin] (kernel 5,
+// zero padding 2).  The network is the learned counterpart of
+// OrthogonalLeastSquaresChannelEstimation's interpolation
+// (hermespy/modem/waveforms/orthogonal/channel.py:29-66): it maps the LS values at
+// a stream's pilots to the channel of every resource element.
+#include <math.h>
+
+#include <algorithm>
+
+#include "hpx_common.hpp"
+
+using namespace hpx;
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kPos = 64;   // pilot positions per sample
+constexpr int kHalo = 2;   // kernel 5
+constexpr int kTaps = 5;
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// GELU, tanh form
+__device__ __forceinline__ float gelu(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  return 0.5f * x * (1.0f + tanhf(k0 * (x + k1 * x * x * x)));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  const float u = k0 * (x + k1 * x * x * x);
+  const float th = tanhf(u);
+  return 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * k0 * (1.0f + 3.0f * k1 * x * x);
+}
+
+// ---------------------------------------------------------------- convolution (MFMA)
+// out[s][p][co] = bias[co] + sum_{k, ci} w[co][k][ci] in[s][p + k - 2][ci]
+// One workgroup per sample; wave v computes rows 32 (v & 1) .. +32 of output column
+// tiles v / 2, v / 2 + 2, ...  mode 0: out_f32 = acc (+ bias), out_bf16 = bf16(that);
+// mode 1: out_f32 = pre-activation, out_bf16 = bf16(gelu(pre)).
+template <int CIN>
+__global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t* __restrict__ in,
+                                                       const uint16_t* __restrict__ w,
+                                                       const float* __restrict__ bias, int cout, int mode,
+                                                       float* __restrict__ out_f32, uint16_t* __restrict__ out_bf16) {
+  constexpr int kStride = CIN + 4;  // padded LDS row: conflict-free 8-byte reads across a half wave
+  __shared__ __align__(16) uint16_t xs[(kPos + 2 * kHalo) * kStride];
+  const int s = blockIdx.x;
+  const int tid = threadIdx.x;
+  const uint16_t* xin = in + static_cast<int64_t>(s) * kPos * CIN;
+  for (int i = tid; i < (kPos + 2 * kHalo) * CIN; i += 256) {
+    const int row = i / CIN, c = i - row * CIN;
+    const int p = row - kHalo;
+    xs[row * kStride + c] = (p >= 0 && p < kPos) ? xin[p * CIN + c] : static_cast<uint16_t>(0);
+  }
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63;
+  const int half = wave & 1;
+  const int m = lane & 31, kq = lane >> 5;
+  for (int ct = wave >> 1; ct < cout / 32; ct += 2) {
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    const int co = ct * 32 + m;
+    const uint16_t* wrow = w + static_cast<int64_t>(co) * kTaps * CIN + 4 * kq;
+    const uint16_t* arow = xs + (32 * half + m) * kStride + 4 * kq;
+#pragma unroll
+    for (int k = 0; k < kTaps; ++k) {
+#pragma unroll 4
+      for (int c0 = 0; c0 < CIN; c0 += 8) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(arow + k * kStride + c0);
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(wrow + k * CIN + c0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+      }
+    }
+    const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int p = 32 * half + (r >> 2) * 8 + kq * 4 + (r & 3);
+      const int64_t o = (static_cast<int64_t>(s) * kPos + p) * cout + co;
+      const float v = acc[r] + bv;
+      if (out_f32) out_f32[o] = v;
+      if (out_bf16) out_bf16[o] = f2bf(mode == 1 ? gelu(v) : v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void __launch_bounds__(256) gelu_bwd_kernel(const float* __restrict__ grad, const float* __restrict__ pre,
+                                                       int64_t n, uint16_t* __restrict__ dz) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dz[i] = f2bf(grad[i] * gelu_grad(pre[i]));
+}
+
+__global__ void __launch_bounds__(256) to_bf16_kernel(const float* __restrict__ x, int64_t n, uint16_t* __restrict__ y) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+// ---------------------------------------------------------------- weight gradient (float32 VALU)
+// part[chunk][co][k][ci] = sum over the chunk's rows of dz[s][p][co] in[s][p + k - 2][ci]
+// part_bias[chunk][co] = sum dz.  Workgroup: one chunk of samples x a 32 (co) x 64 (k, ci) tile.
+constexpr int kWgSamples = 32;
+
+__global__ void __launch_bounds__(256) wgrad_kernel(const uint16_t* __restrict__ dz, const uint16_t* __restrict__ in,
+                                                    int num_samples, int cout, int cin, float* __restrict__ part,
+                                                    float* __restrict__ part_bias) {
+  __shared__ float dzs[kPos][33];
+  __shared__ float xs[kPos + 2 * kHalo][65];
+  const int chunk = blockIdx.x;
+  const int co0 = blockIdx.y * 32;
+  const int kcols = kTaps * cin;
+  const int kc0 = blockIdx.z * 64;
+  const int tid = threadIdx.x;
+  // thread: co = co0 + (tid & 31), columns kc0 + (tid >> 5) + 8 j, j = 0..7
+  const int tco = tid & 31, tcol = tid >> 5;
+  float acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = 0.0f;
+  float accb = 0.0f;
+  const int s_end = std::min(num_samples, (chunk + 1) * kWgSamples);
+  for (int s = chunk * kWgSamples; s < s_end; ++s) {
+    for (int i = tid; i < kPos * 32; i += 256) {
+      const int p = i >> 5, c = i & 31;
+      dzs[p][c] = co0 + c < cout ? bf2f(dz[(static_cast<int64_t>(s) * kPos + p) * cout + co0 + c]) : 0.0f;
+    }
+    // input columns kc0 .. kc0 + 63 of the im2col row: (k, ci) = divmod(col, cin)
+    for (int i = tid; i < (kPos + 2 * kHalo) * 64; i += 256) {
+      const int row = i >> 6, c = i & 63;
+      xs[row][c] = 0.0f;
+    }
+    __syncthreads();
+    for (int i = tid; i < kPos * 64; i += 256) {
+      const int p = i >> 6, c = i & 63;
+      const int col = kc0 + c;
+      if (col < kcols) {
+        const int k = col / cin, ci = col - k * cin;
+        const int q = p + k - kHalo;  // input position feeding output p through tap k
+        xs[p][c] = (q >= 0 && q < kPos) ? bf2f(in[(static_cast<int64_t>(s) * kPos + q) * cin + ci]) : 0.0f;
+      }
+    }
+    __syncthreads();
+    for (int p = 0; p < kPos; ++p) {
+      const float g = dzs[p][tco];
+      if (tcol == 0) accb += g;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] = fmaf(g, xs[p][tcol + 8 * j], acc[j]);
+    }
+    __syncthreads();
+  }
+  const int co = co0 + tco;
+  if (co < cout) {
+    float* prow = part + (static_cast<int64_t>(chunk) * cout + co) * kcols;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int col = kc0 + tcol + 8 * j;
+      if (col < kcols) prow[col] = acc[j];
+    }
+    if (tcol == 0 && blockIdx.z == 0) part_bias[static_cast<int64_t>(chunk) * cout + co] = accb;
+  }
+}
+
+__global__ void __launch_bounds__(256) reduce_parts_kernel(const float* __restrict__ part, int chunks, int64_t n,
+                                                           float* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int c = 0; c < chunks; ++c) s += part[static_cast<int64_t>(c) * n + i];
+  out[i] = s;
+}
+
+// ---------------------------------------------------------------- pilots in, grid out
+// in[s][p][c]: c = 2 v + {0: re, 1: im} of LS value v * 64 + p (v = pilot word), c = 2 V: +-1 stream flag
+__global__ void __launch_bounds__(256) net_input_kernel(const float2* __restrict__ ls, int64_t num_samples,
+                                                        int num_drops, int num_rx, int pilot_words, int cin,
+                                                        uint16_t* __restrict__ x) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= num_samples * kPos) return;
+  const int64_t s = i / kPos;
+  const int p = static_cast<int>(i - s * kPos);
+  const int t = static_cast<int>(s / (static_cast<int64_t>(num_drops) * num_rx));
+  const float2* row = ls + s * pilot_words * kPos;
+  uint16_t* o = x + i * cin;
+  for (int v = 0; v < pilot_words; ++v) {
+    const float2 h = row[v * kPos + p];
+    o[2 * v] = f2bf(h.x);
+    o[2 * v + 1] = f2bf(h.y);
+  }
+  o[2 * pilot_words] = f2bf(t ? 1.0f : -1.0f);
+  for (int c = 2 * pilot_words + 1; c < cin; ++c) o[c] = 0;
+}
+
+// out[s][p][c], c = (w * B + u) * 2 + {re, im} -> est[s][w][B p + u]  (B = N / 64 subcarriers per position)
+__global__ void __launch_bounds__(256) net_output_kernel(const float* __restrict__ out, int64_t num_samples, int words,
+                                                         int n, float2* __restrict__ est) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t wn = static_cast<int64_t>(words) * n;
+  if (i >= num_samples * wn) return;
+  const int64_t s = i / wn;
+  const int e = static_cast<int>(i - s * wn);
+  const int w = e / n, q = e - w * n;
+  const int blk = n / kPos;
+  const int p = q / blk, u = q - p * blk;
+  const int cout = 2 * words * blk;
+  const float* o = out + (s * kPos + p) * cout + 2 * (w * blk + u);
+  est[i] = make_float2(o[0], o[1]);
+}
+
+// ---------------------------------------------------------------- NMSE against the ideal CSI
+// H[s][w][q] = S[d][r][t] c[d][w][q];  acc[0] += |est - H|^2, acc[1] += |H|^2 (per drop when per_drop)
+__device__ __forceinline__ float2 ideal_h(const float2* csi, const float2* spatial, int64_t s, int64_t e, int num_drops,
+                                          int num_rx, int num_tx, int64_t wn) {
+  const int64_t dr = s % (static_cast<int64_t>(num_drops) * num_rx);
+  const int t = static_cast<int>(s / (static_cast<int64_t>(num_drops) * num_rx));
+  const int64_t d = dr / num_rx;
+  const int r = static_cast<int>(dr - d * num_rx);
+  return cmul(spatial[(d * num_rx + r) * num_tx + t], csi[d * wn + e]);
+}
+
+__global__ void __launch_bounds__(256) estimate_error_kernel(const float2* __restrict__ est,
+                                                             const float2* __restrict__ csi,
+                                                             const float2* __restrict__ spatial, int num_drops,
+                                                             int num_rx, int num_tx, int64_t wn,
+                                                             double* __restrict__ err, double* __restrict__ pow) {
+  // one workgroup per (sample); per-drop sums
+  __shared__ double red[2][4];
+  const int64_t s = blockIdx.x;
+  double e2 = 0.0, h2 = 0.0;
+  for (int64_t e = threadIdx.x; e < wn; e += blockDim.x) {
+    const float2 h = ideal_h(csi, spatial, s, e, num_drops, num_rx, num_tx, wn);
+    const float2 x = est[s * wn + e];
+    e2 += static_cast<double>(cabs2(csub(x, h)));
+    h2 += static_cast<double>(cabs2(h));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    e2 += __shfl_down(e2, off, 64);
+    h2 += __shfl_down(h2, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = e2;
+    red[1][threadIdx.x >> 6] = h2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int64_t d = (s % (static_cast<int64_t>(num_drops) * num_rx)) / num_rx;
+    atomicAdd(err + d, red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+    atomicAdd(pow + d, red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+  }
+}
+
+// loss gradient of the batch NMSE sum |out - H|^2 / sum |H|^2 wrt the network output (the
+// normaliser sum |H|^2 is computed beforehand into norm[0])
+__global__ void __launch_bounds__(256) nmse_grad_kernel(const float* __restrict__ out, const float2* __restrict__ csi,
+                                                        const float2* __restrict__ spatial, int num_drops, int num_rx,
+                                                        int num_tx, int words, int n, const double* __restrict__ norm,
+                                                        float* __restrict__ grad, double* __restrict__ loss) {
+  __shared__ double red[4];
+  const int64_t wn = static_cast<int64_t>(words) * n;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t num_samples = static_cast<int64_t>(num_tx) * num_drops * num_rx;
+  double l = 0.0;
+  if (i < num_samples * wn) {
+    const int64_t s = i / wn;
+    const int e = static_cast<int>(i - s * wn);
+    const int w = e / n, q = e - w * n;
+    const int blk = n / kPos;
+    const int p = q / blk, u = q - p * blk;
+    const int cout = 2 * words * blk;
+    const int64_t o = (s * kPos + p) * cout + 2 * (w * blk + u);
+    const float2 h = ideal_h(csi, spatial, s, e, num_drops, num_rx, num_tx, wn);
+    const float dr = out[o] - h.x, di = out[o + 1] - h.y;
+    const float scale = static_cast<float>(2.0 / norm[0]);
+    grad[o] = scale * dr;
+    grad[o + 1] = scale * di;
+    l = static_cast<double>(dr * dr + di * di) / norm[0];
+  }
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(loss, red[0] + red[1] + red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(256) channel_power_kernel(const float2* __restrict__ csi,
+                                                            const float2* __restrict__ spatial, int num_drops,
+                                                            int num_rx, int num_tx, int64_t wn,
+                                                            double* __restrict__ norm) {
+  __shared__ double red[4];
+  const int64_t total = static_cast<int64_t>(num_tx) * num_drops * num_rx * wn;
+  double acc = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t s = i / wn;
+    acc += static_cast<double>(cabs2(ideal_h(csi, spatial, s, i - s * wn, num_drops, num_rx, num_tx, wn)));
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(norm, red[0] + red[1] + red[2] + red[3]);
+}
+
+// ---------------------------------------------------------------- Adam + weight copies
+// params float32 [cout][5][cin] (+ bias [cout] separately); writes w_bf16 [cout][5][cin] and the
+// transposed, flipped copy wt_bf16 [cin][5][cout] (wt[ci][k][co] = w[co][4 - k][ci]) used by
+// the backward-data convolution.
+__global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                                                   float* __restrict__ m1, float* __restrict__ m2, int64_t n, float lr,
+                                                   float b1, float b2, float eps, float c1, float c2) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.0f - b1) * gi;
+  const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+__global__ void __launch_bounds__(256) weight_copies_kernel(const float* __restrict__ p, int cout, int cin,
+                                                            uint16_t* __restrict__ w, uint16_t* __restrict__ wt) {
+  const int64_t n = static_cast<int64_t>(cout) * kTaps * cin;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int co = static_cast<int>(i / (kTaps * cin));
+  const int rest = static_cast<int>(i - static_cast<int64_t>(co) * kTaps * cin);
+  const int k = rest / cin, ci = rest - k * cin;
+  const uint16_t v = f2bf(p[i]);
+  if (w) w[i] = v;
+  if (wt) wt[(static_cast<int64_t>(ci) * kTaps + (kTaps - 1 - k)) * cout + co] = v;
+}
+
+unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+}  // namespace
+
+extern "C" int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, const uint16_t* weight,
+                                  const float* bias, int cout, int mode, float* out_f32, uint16_t* out_bf16,
+                                  void* stream) {
+  HPX_REQUIRE(num_samples >= 0 && num_samples <= 0x7fffffff, "hpx_conv1d_forward: invalid sample count");
+  HPX_REQUIRE(cout >= 32 && cout % 32 == 0, "hpx_conv1d_forward: cout must be a multiple of 32 (got %d)", cout);
+  HPX_REQUIRE(mode == 0 || mode == 1, "hpx_conv1d_forward: mode must be 0 (linear) or 1 (GELU)");
+  if (num_samples == 0) return HPX_OK;
+  HPX_REQUIRE(in && weight && (out_f32 || out_bf16), "hpx_conv1d_forward: null pointer");
+  const dim3 g(static_cast<unsigned>(num_samples));
+  const hipStream_t s = as_stream(stream);
+  switch (cin) {
+    case 8: hipLaunchKernelGGL(conv_fwd_kernel<8>, g, dim3(256), 0, s, in, weight, bias, cout, mode, out_f32, out_bf16); break;
+    case 64: hipLaunchKernelGGL(conv_fwd_kernel<64>, g, dim3(256), 0, s, in, weight, bias, cout, mode, out_f32, out_bf16); break;
+    case 224: hipLaunchKernelGGL(conv_fwd_kernel<224>, g, dim3(256), 0, s, in, weight, bias, cout, mode, out_f32, out_bf16); break;
+    default:
+      set_error("hpx_conv1d_forward: cin %d not instantiated (8, 64, 224)", cin);
+      return HPX_ERR_UNSUPPORTED;
+  }
+  return check_launch("hpx_conv1d_forward");
+}
+
+extern "C" int64_t hpx_conv1d_wgrad_workspace(int64_t num_samples, int cin, int cout) {
+  const int64_t chunks = (num_samples + kWgSamples - 1) / kWgSamples;
+  return chunks * static_cast<int64_t>(cout) * (kTaps * cin + 1) * static_cast<int64_t>(sizeof(float));
+}
+
+extern "C" int hpx_conv1d_wgrad(const uint16_t* dz, const uint16_t* in, int64_t num_samples, int cin, int cout,
+                                float* workspace, float* grad_weight, float* grad_bias, void* stream) {
+  HPX_REQUIRE(num_samples >= 1 && num_samples <= 0x7fffffff && cin >= 1 && cout >= 1,
+              "hpx_conv1d_wgrad: invalid sizes");
+  HPX_REQUIRE(dz && in && workspace && grad_weight && grad_bias, "hpx_conv1d_wgrad: null pointer");
+  const int chunks = static_cast<int>((num_samples + kWgSamples - 1) / kWgSamples);
+  const int64_t nw = static_cast<int64_t>(cout) * kTaps * cin;
+  float* part_w = workspace;
+  float* part_b = workspace + static_cast<int64_t>(chunks) * nw;
+  const hipStream_t s = as_stream(stream);
+  const dim3 g(static_cast<unsigned>(chunks), static_cast<unsigned>((cout + 31) / 32),
+               static_cast<unsigned>((kTaps * cin + 63) / 64));
+  hipLaunchKernelGGL(wgrad_kernel, g, dim3(256), 0, s, dz, in, static_cast<int>(num_samples), cout, cin, part_w,
+                     part_b);
+  hipLaunchKernelGGL(reduce_parts_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, part_w, chunks, nw, grad_weight);
+  hipLaunchKernelGGL(reduce_parts_kernel, dim3(blocks_for(cout)), dim3(256), 0, s, part_b, chunks,
+                     static_cast<int64_t>(cout), grad_bias);
+  return check_launch("hpx_conv1d_wgrad");
+}
+
+extern "C" int hpx_gelu_backward(const float* grad, const float* pre, int64_t n, uint16_t* dz, void* stream) {
+  HPX_REQUIRE(n >= 0, "hpx_gelu_backward: negative size");
+  if (n == 0) return HPX_OK;
+  HPX_REQUIRE(grad && pre && dz, "hpx_gelu_backward: null pointer");
+  hipLaunchKernelGGL(gelu_bwd_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), grad, pre, n, dz);
+  return check_launch("hpx_gelu_backward");
+}
+
+extern "C" int hpx_to_bf16(const float* x, int64_t n, uint16_t* y, void* stream) {
+  HPX_REQUIRE(n >= 0, "hpx_to_bf16: negative size");
+  if (n == 0) return HPX_OK;
+  HPX_REQUIRE(x && y, "hpx_to_bf16: null pointer");
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), x, n, y);
+  return check_launch("hpx_to_bf16");
+}
+
+extern "C" int hpx_net_input(const hpx_c64* ls, int64_t num_drops, int num_rx, int num_tx, int pilot_words, int cin,
+                             uint16_t* x, void* stream) {
+  HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff && num_rx >= 1 && num_tx >= 1 && pilot_words >= 1 &&
+                  cin >= 2 * pilot_words + 1,
+              "hpx_net_input: invalid sizes");
+  const int64_t samples = static_cast<int64_t>(num_tx) * num_drops * num_rx;
+  if (samples == 0) return HPX_OK;
+  HPX_REQUIRE(ls && x, "hpx_net_input: null pointer");
+  hipLaunchKernelGGL(net_input_kernel, dim3(blocks_for(samples * kPos)), dim3(256), 0, as_stream(stream),
+                     reinterpret_cast<const float2*>(ls), samples, static_cast<int>(num_drops), num_rx, pilot_words,
+                     cin, x);
+  return check_launch("hpx_net_input");
+}
+
+extern "C" int hpx_net_output(const float* out, int64_t num_samples, int words, int num_subcarriers, hpx_c64* est,
+                              void* stream) {
+  HPX_REQUIRE(num_samples >= 0 && words >= 1 && num_subcarriers >= kPos && num_subcarriers % kPos == 0,
+              "hpx_net_output: subcarriers must be a multiple of %d", kPos);
+  if (num_samples == 0) return HPX_OK;
+  HPX_REQUIRE(out && est, "hpx_net_output: null pointer");
+  const int64_t n = num_samples * words * num_subcarriers;
+  hipLaunchKernelGGL(net_output_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), out, num_samples, words,
+                     num_subcarriers, reinterpret_cast<float2*>(est));
+  return check_launch("hpx_net_output");
+}
+
+extern "C" int hpx_estimate_error(const hpx_c64* est, const hpx_c64* csi, const hpx_c64* spatial, int64_t num_drops,
+                                  int num_rx, int num_tx, int64_t grid_elements, double* err, double* power,
+                                  void* stream) {
+  HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff && num_rx >= 1 && num_tx >= 1 && grid_elements >= 1,
+              "hpx_estimate_error: invalid sizes");
+  const int64_t samples = static_cast<int64_t>(num_tx) * num_drops * num_rx;
+  if (samples == 0) return HPX_OK;
+  HPX_REQUIRE(est && csi && spatial && err && power, "hpx_estimate_error: null pointer");
+  HPX_REQUIRE(samples <= 0x7fffffff, "hpx_estimate_error: too many samples");
+  hipLaunchKernelGGL(estimate_error_kernel, dim3(static_cast<unsigned>(samples)), dim3(256), 0, as_stream(stream),
+                     reinterpret_cast<const float2*>(est), reinterpret_cast<const float2*>(csi),
+                     reinterpret_cast<const float2*>(spatial), static_cast<int>(num_drops), num_rx, num_tx,
+                     grid_elements, err, power);
+  return check_launch("hpx_estimate_error");
+}
+
+extern "C" int hpx_nmse_grad(const float* out, const hpx_c64* csi, const hpx_c64* spatial, int64_t num_drops,
+                             int num_rx, int num_tx, int words, int num_subcarriers, double* norm, float* grad,
+                             double* loss, void* stream) {
+  HPX_REQUIRE(num_drops >= 1 && num_drops <= 0x7fffffff && num_rx >= 1 && num_tx >= 1 && words >= 1 &&
+                  num_subcarriers % kPos == 0,
+              "hpx_nmse_grad: invalid sizes");
+  HPX_REQUIRE(out && csi && spatial && norm && grad && loss, "hpx_nmse_grad: null pointer");
+  const int64_t wn = static_cast<int64_t>(words) * num_subcarriers;
+  const int64_t n = static_cast<int64_t>(num_tx) * num_drops * num_rx * wn;
+  const hipStream_t s = as_stream(stream);
+  const auto* c = reinterpret_cast<const float2*>(csi);
+  const auto* sp = reinterpret_cast<const float2*>(spatial);
+  // norm[0] and loss[0] must be zero on entry
+  hipLaunchKernelGGL(channel_power_kernel, dim3(static_cast<unsigned>(std::min<int64_t>(blocks_for(n), 2048))),
+                     dim3(256), 0, s, c, sp, static_cast<int>(num_drops), num_rx, num_tx, wn, norm);
+  hipLaunchKernelGGL(nmse_grad_kernel, dim3(blocks_for(n)), dim3(256), 0, s, out, c, sp, static_cast<int>(num_drops),
+                     num_rx, num_tx, words, num_subcarriers, norm, grad, loss);
+  return check_launch("hpx_nmse_grad");
+}
+
+extern "C" int hpx_adam_step(float* param, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1,
+                             float beta2, float eps, int64_t step, void* stream) {
+  HPX_REQUIRE(n >= 0 && step >= 1, "hpx_adam_step: invalid arguments");
+  if (n == 0) return HPX_OK;
+  HPX_REQUIRE(param && grad && m1 && m2, "hpx_adam_step: null pointer");
+  const float c1 = static_cast<float>(1.0 - pow(static_cast<double>(beta1), static_cast<double>(step)));
+  const float c2 = static_cast<float>(1.0 - pow(static_cast<double>(beta2), static_cast<double>(step)));
+  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), param, grad, m1, m2, n, lr,
+                     beta1, beta2, eps, c1, c2);
+  return check_launch("hpx_adam_step");
+}
+
+extern "C" int hpx_conv1d_weight_copies(const float* param, int cout, int cin, uint16_t* weight,
+                                        uint16_t* weight_transposed, void* stream) {
+  HPX_REQUIRE(cout >= 1 && cin >= 1, "hpx_conv1d_weight_copies: invalid sizes");
+  HPX_REQUIRE(param && (weight || weight_transposed), "hpx_conv1d_weight_copies: null pointer");
+  const int64_t n = static_cast<int64_t>(cout) * kTaps * cin;
+  hipLaunchKernelGGL(weight_copies_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), param, cout, cin,
+                     weight, weight_transposed);
+  return check_launch("hpx_conv1d_weight_copies");
+}

@@ -65,25 +65,60 @@ class BatchedLink:
             self.corr_tx = torch.as_tensor(np.ascontiguousarray(rtx), device=dev)
         else:
             self.corr_rx = self.corr_tx = None
+        self.c_frame = None
+        if link.frame_mask is not None:
+            mask = link.frame_mask
+            data, ref = mask == 1, mask == 0
+            el = np.full(mask.shape, _lib.ELEMENT_NULL, dtype=np.int64)
+            el[data] = (np.cumsum(data, axis=1) - 1)[data]  # data index within the word
+            el[ref] = -1 - np.arange(int(ref.sum()))  # reference elements in frame (row-major) order
+            per_word = data.sum(axis=1)
+            word_data = np.concatenate([[0], np.cumsum(per_word)]).astype(np.int32)
+            self.elements = torch.as_tensor(el.astype(np.int32), device=dev)
+            self.word_data = torch.as_tensor(word_data, device=dev)
+            self.pilots = (torch.as_tensor(np.asarray(link.pilots, dtype=np.complex64), device=dev)
+                           if link.num_references else None)
+            self.c_frame = _lib.HpxFrame(_lib.ptr(self.elements), _lib.ptr(self.word_data), _lib.ptr(self.pilots),
+                                         link.num_references, int(word_data[-1]), int(per_word.max()))
 
     def run(self, seed: int, first_drop: int, noise_power: torch.Tensor, errors: torch.Tensor,
-            equalized: torch.Tensor | None = None, stream: int | None = None) -> None:
-        """Launch ``noise_power.numel()`` drops (asynchronous on the current stream)."""
+            equalized: torch.Tensor | None = None, stream: int | None = None, doppler=None,
+            grid: torch.Tensor | None = None, csi: torch.Tensor | None = None,
+            spatial: torch.Tensor | None = None) -> None:
+        """Launch ``noise_power.numel()`` drops (asynchronous on the current stream).
 
+        doppler: per-drop (NLOS, LOS) doppler_frequency, (num_drops, 2) float64 (host array or device
+        tensor), bounded by the link's Doppler.  grid / csi / spatial: optional complex64 exports of the
+        received grid (drops, Nrx, words, N), the ideal-CSI factor c (drops, words, N) and the spatial
+        response S (drops, Nrx, Ntx); the ideal CSI of element (w, k) is S c[w, k]."""
+
+        lk = self.link
         n = int(noise_power.numel())
         if errors.numel() < n or errors.dtype != torch.int32:
             raise ValueError("errors must be an int32 tensor with one entry per drop")
         if noise_power.dtype != torch.float64:
             raise ValueError("noise_power must be float64")
-        if equalized is not None:
-            need = n * self.link.num_tx * self.link.words * self.link.num_subcarriers
-            if equalized.dtype != torch.complex64 or equalized.numel() < need:
-                raise ValueError("equalized must be complex64 with num_drops * num_tx * words * N entries")
+        sizes = {"equalized": (equalized, lk.num_tx * lk.num_data), "grid": (grid, lk.num_rx * lk.words * lk.num_subcarriers),
+                 "csi": (csi, lk.words * lk.num_subcarriers), "spatial": (spatial, lk.num_rx * lk.num_tx)}
+        for name, (t, per_drop) in sizes.items():
+            if t is not None and (t.dtype != torch.complex64 or not t.is_contiguous() or t.numel() < n * per_drop):
+                raise ValueError(f"{name} must be a contiguous complex64 tensor with {per_drop} entries per drop")
+        if doppler is not None:
+            if isinstance(doppler, np.ndarray):
+                bound = np.array([lk.doppler, lk.doppler if lk.los_doppler is None else lk.los_doppler])
+                if doppler.shape != (n, 2) or np.any(np.abs(doppler) > bound * (1 + 1e-12)):
+                    raise ValueError("per-drop Doppler must be (num_drops, 2) and bounded by the link's Doppler")
+                doppler = torch.as_tensor(np.ascontiguousarray(doppler, dtype=np.float64), device=self.device)
+            elif doppler.dtype != torch.float64 or doppler.numel() != 2 * n:
+                raise ValueError("per-drop Doppler must be a float64 (num_drops, 2) tensor")
+            self._doppler = doppler  # held until the next launch
         lib = _lib.load()
-        st = lib.hpx_run_batch(ctypes.byref(self.c_link), _lib.ptr(self.delays), _lib.ptr(self.path_gains),
-                               _lib.ptr(self.corr_rx), _lib.ptr(self.corr_tx), ctypes.c_uint64(seed & (2**64 - 1)),
-                               int(first_drop), n, _lib.ptr(noise_power), _lib.ptr(errors), _lib.ptr(equalized),
-                               _lib.stream_handle() if stream is None else stream)
+        out = _lib.HpxBatchOut(_lib.ptr(errors), _lib.ptr(equalized), _lib.ptr(grid), _lib.ptr(csi), _lib.ptr(spatial))
+        st = lib.hpx_run_batch_frames(
+            ctypes.byref(self.c_link), ctypes.byref(self.c_frame) if self.c_frame is not None else None,
+            _lib.ptr(self.delays), _lib.ptr(self.path_gains), _lib.ptr(self.corr_rx), _lib.ptr(self.corr_tx),
+            _lib.ptr(doppler), ctypes.c_uint64(seed & (2**64 - 1)), int(first_drop), n, _lib.ptr(noise_power),
+            ctypes.byref(out), _lib.stream_handle() if stream is None else stream)
         _lib.check(st)
 
 

@@ -1,0 +1,206 @@
+"""Heterogeneous batches: thousands of drops with different antenna counts, FFT
+sizes, path counts, power-delay profiles, Doppler, modulation and SNR in one
+graph-captured submission.
+
+The reference simulates one configuration at a time: every shape is its own
+``Simulation`` whose ``MonteCarloActor`` loops over drops
+(``hermespy/core/pymonte/actors.py:316-452``).  Here the host sorts a mixed
+population into kernel classes (antennas, threads per drop, samples per
+thread), writes one plan per drop (``hpx_uneven_prepare``) and replays one
+launch per class from a captured graph (``hpx_uneven_run``); drop i's result
+depends only on (seed, its drop index, its link), exactly as in
+``hpx_run_batch``.  A per-class autotuner picks the workgroup size.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .configs import LinkConfig
+
+MAX_CLASSES = 64
+SUBCARRIER_SPACING = 30e3  # sampling rate = N x 30 kHz
+
+
+@dataclass
+class DropShape:
+    """One drop of a mixed population."""
+
+    antennas: int
+    num_subcarriers: int
+    num_paths: int
+    decay_db: float
+    doppler_hz: float
+    modulation_order: int
+    snr_db: float
+
+    def link(self) -> LinkConfig:
+        n = self.num_subcarriers
+        fs = n * SUBCARRIER_SPACING
+        p = 10.0 ** (-self.decay_db * np.arange(self.num_paths) / 10.0)
+        return LinkConfig(self.antennas, self.antennas, n, 14, 1.0 / 8.0, self.modulation_order, fs,
+                          np.arange(self.num_paths) / fs, p / p.sum(), np.zeros(self.num_paths),
+                          2 * np.pi * self.doppler_hz, detector="mmse", name="uneven")
+
+
+def draw_population(num_drops: int, seed: int = 20261202, exclude=((8, 2048),)) -> list:
+    """The mixed population: antennas {1, 2, 4, 8} square, N in {128, ..., 2048} with CP N/8, 5..23
+    sample-spaced paths with an exponential profile of 1..6 dB per tap, Doppler 10..900 Hz,
+    QPSK / 16-QAM / 64-QAM, SNR from 0..30 dB in 3 dB steps.  Shapes in `exclude` (antennas, N)
+    are redrawn: a 8x8 drop at 2048 subcarriers needs more than one CU's 160 KB of LDS."""
+
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < num_drops:
+        a = int(rng.choice([1, 2, 4, 8]))
+        n = int(rng.choice([128, 256, 512, 1024, 2048]))
+        shape = DropShape(a, n, int(rng.integers(5, 24)), float(rng.uniform(1.0, 6.0)), float(rng.uniform(10.0, 900.0)),
+                          int(rng.choice([4, 16, 64])), float(rng.choice(np.arange(0.0, 31.0, 3.0))))
+        if (a, n) in exclude:
+            continue
+        out.append(shape)
+    return out
+
+
+def _c_link(cfg: LinkConfig, workgroup_size: int = 0) -> _lib.HpxLink:
+    return _lib.HpxLink(cfg.num_rx, cfg.num_tx, cfg.num_subcarriers, cfg.words, cfg.prefix, cfg.modulation_order,
+                        cfg.num_paths, cfg.num_sinusoids, cfg.max_delay, _lib.HPX_DETECT_MMSE,
+                        int(bool(cfg.dc_suppression)), float(cfg.bandwidth), float(cfg.doppler),
+                        float(cfg.doppler if cfg.los_doppler is None else cfg.los_doppler), float(cfg.gain),
+                        float(cfg.device_power), int(workgroup_size))
+
+
+class UnevenBatch:
+    """Device-resident plans of one mixed population (drop k of the population is Philox drop
+    first_drop + k)."""
+
+    def __init__(self, shapes: list, seed: int = 20261202, first_drop: int = 0, device="cuda",
+                 workgroup_sizes: dict | None = None) -> None:
+        self.shapes = shapes
+        self.seed = int(seed)
+        self.first_drop = int(first_drop)
+        self.device = torch.device(device)
+        self.lib = _lib.load()
+        self.links = [s.link() for s in shapes]
+        n = len(shapes)
+        dev = self.device
+        # per-drop path tables, concatenated
+        lens = np.array([c.num_paths for c in self.links])
+        offs = np.concatenate([[0], np.cumsum(lens)[:-1]])
+        delays = np.concatenate([c.delay_samples for c in self.links]).astype(np.int32)
+        gains = np.concatenate([c.path_gains.reshape(-1) for c in self.links])
+        self.delays = torch.as_tensor(delays, device=dev)
+        self.gains = torch.as_tensor(gains, dtype=torch.float64, device=dev)
+        self.tables = (_lib.HpxDropTables * n)()
+        for i in range(n):
+            self.tables[i].delays = self.delays.data_ptr() + 4 * int(offs[i])
+            self.tables[i].path_gains = self.gains.data_ptr() + 8 * 2 * int(offs[i])
+        self.noise_power = torch.as_tensor([c.noise_power(s.snr_db) for c, s in zip(self.links, shapes)],
+                                           dtype=torch.float64, device=dev)
+        self.errors = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.bits = np.array([c.bits_per_drop for c in self.links], dtype=np.int64)
+        self.prepare(workgroup_sizes or {})
+
+    def class_key(self, i: int) -> tuple:
+        return (self.links[i].num_tx, self.links[i].num_subcarriers)
+
+    def prepare(self, workgroup_sizes: dict) -> None:
+        """(Re)build the plans; workgroup_sizes maps (antennas, N) to a workgroup size (0: default)."""
+
+        n = len(self.links)
+        self.workgroup_sizes = dict(workgroup_sizes)
+        links = (_lib.HpxLink * n)(*[_c_link(c, self.workgroup_sizes.get(self.class_key(i), 0))
+                                     for i, c in enumerate(self.links)])
+        idx = np.arange(self.first_drop, self.first_drop + n, dtype=np.uint32)
+        plan_bytes = int(self.lib.hpx_uneven_plan_bytes())
+        plans = np.empty(n * plan_bytes, dtype=np.uint8)
+        order = np.empty(n, dtype=np.int64)
+        classes = (_lib.HpxUnevenClass * MAX_CLASSES)()
+        nc = _lib.c_int(0)
+        _lib.check(self.lib.hpx_uneven_prepare(links, self.tables, idx.ctypes.data, n, ctypes.c_uint64(self.seed),
+                                               self.noise_power.data_ptr(), self.errors.data_ptr(), plans.ctypes.data,
+                                               order.ctypes.data, classes, MAX_CLASSES, ctypes.byref(nc)))
+        self.plans = torch.as_tensor(plans, device=self.device)
+        self.order = order
+        self.num_classes = nc.value
+        self.classes = classes
+        self.graph = None
+
+    def run(self) -> None:
+        _lib.check(self.lib.hpx_uneven_run(self.plans.data_ptr(), self.classes, self.num_classes,
+                                           _lib.stream_handle()))
+
+    def run_class(self, k: int) -> None:
+        _lib.check(self.lib.hpx_uneven_run(self.plans.data_ptr(), ctypes.byref(self.classes[k]), 1,
+                                           _lib.stream_handle()))
+
+    def capture(self) -> None:
+        """Capture every class launch into one graph (after one eager run has set the kernels' LDS limits)."""
+
+        self.run()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        self.graph = g
+
+    def replay(self) -> None:
+        if self.graph is None:
+            self.capture()
+        self.graph.replay()
+
+    def class_table(self) -> list:
+        out = []
+        for k in range(self.num_classes):
+            c = self.classes[k]
+            i = int(self.order[c.first])
+            out.append({"antennas": c.num_tx, "num_subcarriers": self.links[i].num_subcarriers, "drops": c.count,
+                        "threads": c.threads, "pairs": c.pairs, "lds_bytes": c.lds_bytes})
+        return out
+
+    def ber_by_snr(self) -> dict:
+        err = self.errors.cpu().numpy().astype(np.int64)
+        snr = np.array([s.snr_db for s in self.shapes])
+        return {float(v): float(err[snr == v].sum() / self.bits[snr == v].sum()) for v in np.unique(snr)}
+
+
+def workgroup_candidates(n: int) -> list:
+    """Workgroup sizes with 1, 2 or 4 sample pairs per thread, 64..512 threads."""
+
+    return [t for t in (n // 2, n // 4, n // 8) if 64 <= t <= 512]
+
+
+def tune(batch: UnevenBatch, reps: int = 3, verbose: bool = False) -> dict:
+    """Per (antennas, N) class, the workgroup size with the shortest launch; re-prepares the batch."""
+
+    keys = sorted({batch.class_key(i) for i in range(len(batch.links))})
+    best = {}
+    timings = {}
+    for key in keys:
+        cands = workgroup_candidates(key[1])
+        times = {}
+        for t in cands:
+            batch.prepare({**best, key: t})
+            k = next(j for j in range(batch.num_classes)
+                     if batch.class_key(int(batch.order[batch.classes[j].first])) == key)
+            batch.run_class(k)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                batch.run_class(k)
+            torch.cuda.synchronize()
+            times[t] = (time.perf_counter() - t0) / reps
+        best[key] = min(times, key=times.get)
+        timings[key] = times
+        if verbose:
+            print(f"tune {key}: " + ", ".join(f"{t}: {1e3 * v:.3f} ms" for t, v in times.items()) + f" -> {best[key]}",
+                  flush=True)
+    batch.prepare(best)
+    return {"choice": {f"{a}x{a}-{n}": t for (a, n), t in best.items()},
+            "ms": {f"{a}x{a}-{n}": {str(t): 1e3 * v for t, v in tt.items()} for (a, n), tt in timings.items()}}

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define HPX_ABI_VERSION 1
+#define HPX_ABI_VERSION 2
 
 typedef enum hpx_status {
   HPX_OK = 0,
@@ -152,6 +152,35 @@ int hpx_ls_estimate(const hpx_c64* grid, int num_streams, int64_t grid_size, int
                     int num_fill, const int32_t* fill_index, const int32_t* fill_source, hpx_c64* estimate,
                     void* stream);
 
+/* Least squares at every stream's own pilots, y_p / x_p
+ * (hermespy/modem/waveforms/orthogonal/channel.py:44-48, per transmit stream):
+ * ls[t][d][r][j] = grid[d][r][pilot_index[t][j]] / pilot_values[t][j] for grids
+ * [num_drops][num_rx][grid_elements]. */
+int hpx_pilot_ls(const hpx_c64* grid, int64_t num_drops, int num_rx, int num_tx, int64_t grid_elements,
+                 int num_pilots, const int32_t* pilot_index, const hpx_c64* pilot_values, hpx_c64* ls,
+                 void* stream);
+
+/* Linear MMSE interpolation of pilot LS values over the grid with one precomputed
+ * filter set: est[t][rows[i]][e] = sum_j filter[t][e][j] ls[t][rows[i]][j] for the
+ * `count` listed rows (row = drop * num_rx + r) of ls [num_tx][num_rows][num_pilots];
+ * est is [num_tx][num_rows][elements].  The fallback estimator of the learned one;
+ * the reference has no LMMSE estimator (its OFDM estimators are ideal CSI and LS,
+ * channel_estimation.py:215-295, orthogonal/channel.py:25-66). */
+int hpx_lmmse_apply(const hpx_c64* ls, int num_tx, int64_t num_rows, int num_pilots, const hpx_c64* filter,
+                    int64_t elements, const int32_t* rows, int count, hpx_c64* est, void* stream);
+
+/* Detection of every data element from estimated channels est [num_tx][num_drops][num_rx][words * N]
+ * and received grids grid [num_drops][num_rx][words * N] (ZF or MMSE with noise_power[d]), hard
+ * demapping and errors[d] = bit errors of drop first_drop + d against its Philox bits.
+ * elements / word_data as in hpx_frame (both NULL: all-data frame).  equalized
+ * [num_drops][num_tx][num_data] or NULL.  Replaces ChannelEqualization.equalize_channel +
+ * unmap + BitErrorEvaluator.evaluate on estimated CSI (waveform.py:235-250,
+ * psk_qam_mapping.py:171-259, evaluators.py:239-258). */
+int hpx_detect_estimated(const hpx_c64* grid, const hpx_c64* est, int64_t num_drops, int num_rx, int num_tx,
+                         int words, int num_subcarriers, const int32_t* elements, const int32_t* word_data,
+                         int num_data, int modulation_order, int detector, const double* noise_power, uint64_t seed,
+                         int64_t first_drop, int32_t* errors, hpx_c64* equalized, void* stream);
+
 /* Per resource element equalisation: y [num_rx][K], h [num_rx][num_tx][K] -> x [num_tx][K].
  * HPX_DETECT_ZF: pseudo-inverse (num_rx == 1: y / sum_t h), replaces
  * ZeroForcingChannelEqualization.equalize_channel, hermespy/modem/waveform.py:235-250.
@@ -202,6 +231,142 @@ int hpx_run_batch(const hpx_link* link_host, const int32_t* delays, const double
                   const hpx_c128* corr_rx, const hpx_c128* corr_tx, uint64_t seed, int64_t first_drop,
                   int64_t num_drops, const double* noise_power, int32_t* errors, hpx_c64* equalized,
                   void* stream);
+
+/* Element value of a null (unused) resource element in hpx_frame.elements. */
+#define HPX_ELEMENT_NULL (-2147483647 - 1)
+
+/* OFDM frame with reference and null elements for hpx_run_batch_frames: the
+ * reference's resource mask (hermespy/modem/waveforms/orthogonal/waveform.py:1195-1211)
+ * with its place/pick order, row-major over (word, subcarrier) (:361-406,1222-1290).
+ * Reference k of stream t transmits pilots[t][k]; streams whose pilot is zero
+ * stay silent there (orthogonal pilots for per-stream estimation; the reference
+ * places the same pilots on every stream, which a zero-free table reproduces). */
+typedef struct hpx_frame {
+  const int32_t* elements;  /* int32 [words][N]: >= 0 data symbol index within its word,
+                               -1 - k reference element k, or HPX_ELEMENT_NULL */
+  const int32_t* word_data; /* int32 [words + 1]: data symbols preceding each word */
+  const hpx_c64* pilots;    /* [num_tx][num_references] */
+  int32_t num_references;
+  int32_t num_data;       /* word_data[words] */
+  int32_t max_word_data;  /* largest data symbol count of one word */
+} hpx_frame;
+
+/* Outputs of hpx_run_batch_frames (device arrays; every one but errors may be NULL). */
+typedef struct hpx_batch_out {
+  int32_t* errors;    /* int32 [num_drops] bit errors over the data elements */
+  hpx_c64* equalized; /* [num_drops][num_tx][num_data] detected data symbols */
+  hpx_c64* grid;      /* [num_drops][num_rx][words][N] received grid: CP removal, FFT, 1/sqrt(N) (ofdm.py:132-161) */
+  hpx_c64* csi;       /* [num_drops][words][N] c of the ideal CSI H[r][t][w][k] = S[r][t] c[w][k]
+                         (channel_estimation.py:215-295) */
+  hpx_c64* spatial;   /* [num_drops][num_rx][num_tx] spatial response S */
+} hpx_batch_out;
+
+/* hpx_run_batch over a frame with reference/null elements (NULL frame: all data),
+ * with per-drop Doppler and optional exports of the received grid and the ideal
+ * CSI (the inputs of pilot-based estimators, orthogonal/channel.py:29-66).
+ * doppler: float64 [num_drops][2] {NLOS, LOS} doppler_frequency per drop, or NULL
+ * for the link's; |values| must not exceed the link's doppler / los_doppler, which
+ * size the Taylor blocks of the taps. */
+int hpx_run_batch_frames(const hpx_link* link_host, const hpx_frame* frame_host, const int32_t* delays,
+                         const double* path_gains, const hpx_c128* corr_rx, const hpx_c128* corr_tx,
+                         const double* doppler, uint64_t seed, int64_t first_drop, int64_t num_drops,
+                         const double* noise_power, const hpx_batch_out* out_host, void* stream);
+
+/* ---------------------------------------------------------------- learned estimator
+ * A 1-D convolutional network over the 64 pilot positions of each (transmit,
+ * receive) stream pair, replacing the interpolation of
+ * OrthogonalLeastSquaresChannelEstimation (orthogonal/channel.py:29-66); no reference
+ * counterpart otherwise.  Activations are bfloat16 (uint16 bit patterns)
+ * [samples][64][channels], sample = (t * num_drops + d) * num_rx + r; weights
+ * bfloat16 [cout][5][cin], kernel 5 with zero padding 2. */
+
+/* out[s][p][co] = bias[co] + sum_{k, ci} weight[co][k][ci] in[s][p + k - 2][ci] on the MFMA units
+ * (float32 accumulation).  mode 0: out_f32 = that, out_bf16 = bf16(that); mode 1 (GELU, tanh form):
+ * out_f32 = the pre-activation, out_bf16 = bf16(gelu(pre)).  bias may be NULL; either output may be
+ * NULL.  cin in {8, 64, 224}, cout a multiple of 32. */
+int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, const uint16_t* weight, const float* bias,
+                       int cout, int mode, float* out_f32, uint16_t* out_bf16, void* stream);
+
+/* float32 workspace bytes of hpx_conv1d_wgrad. */
+int64_t hpx_conv1d_wgrad_workspace(int64_t num_samples, int cin, int cout);
+
+/* grad_weight[co][k][ci] = sum_{s, p} dz[s][p][co] in[s][p + k - 2][ci], grad_bias[co] = sum dz (float32). */
+int hpx_conv1d_wgrad(const uint16_t* dz, const uint16_t* in, int64_t num_samples, int cin, int cout, float* workspace,
+                     float* grad_weight, float* grad_bias, void* stream);
+
+/* dz = bf16(grad * gelu'(pre)) elementwise. */
+int hpx_gelu_backward(const float* grad, const float* pre, int64_t n, uint16_t* dz, void* stream);
+
+/* y = bf16(x), round to nearest even. */
+int hpx_to_bf16(const float* x, int64_t n, uint16_t* y, void* stream);
+
+/* Network input from pilot LS values ls [num_tx][num_drops][num_rx][pilot_words * 64] (hpx_pilot_ls):
+ * x[s][p][2 v + {0, 1}] = {re, im} ls[..][v * 64 + p], x[s][p][2 pilot_words] = -1 / +1 for stream
+ * 0 / others, zero up to cin channels. */
+int hpx_net_input(const hpx_c64* ls, int64_t num_drops, int num_rx, int num_tx, int pilot_words, int cin,
+                  uint16_t* x, void* stream);
+
+/* Channel estimates est [num_samples][words * N] from the network output out [num_samples][64][2 words B]
+ * (B = N / 64 subcarriers per position): est[s][w][B p + u] = out[s][p][2 (w B + u) + {0, 1}]. */
+int hpx_net_output(const float* out, int64_t num_samples, int words, int num_subcarriers, hpx_c64* est,
+                   void* stream);
+
+/* Per-drop squared estimation error and channel power against the ideal CSI H[r][t] = S[r][t] c:
+ * err[d] += sum |est - H|^2, power[d] += sum |H|^2 (float64; zero them first).  est as in
+ * hpx_lmmse_apply, csi / spatial as exported by hpx_run_batch_frames. */
+int hpx_estimate_error(const hpx_c64* est, const hpx_c64* csi, const hpx_c64* spatial, int64_t num_drops,
+                       int num_rx, int num_tx, int64_t grid_elements, double* err, double* power, void* stream);
+
+/* Batch NMSE loss sum |out - H|^2 / sum |H|^2 of network outputs and its gradient grad (same layout
+ * as out); norm[0] and loss[0] (float64) must be zero on entry and receive sum |H|^2 and the loss. */
+int hpx_nmse_grad(const float* out, const hpx_c64* csi, const hpx_c64* spatial, int64_t num_drops, int num_rx,
+                  int num_tx, int words, int num_subcarriers, double* norm, float* grad, double* loss, void* stream);
+
+/* One Adam step (bias-corrected, step >= 1) on float32 parameters. */
+int hpx_adam_step(float* param, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1, float beta2,
+                  float eps, int64_t step, void* stream);
+
+/* bfloat16 copies of float32 weights [cout][5][cin]: weight (same layout) and weight_transposed
+ * [cin][5][cout] with weight_transposed[ci][k][co] = w[co][4 - k][ci] (the backward-data
+ * convolution's weights); either may be NULL. */
+int hpx_conv1d_weight_copies(const float* param, int cout, int cin, uint16_t* weight, uint16_t* weight_transposed,
+                             void* stream);
+
+/* ---------------------------------------------------------------- heterogeneous batches
+ * Thousands of drops with different antenna counts, FFT sizes, path counts, Doppler,
+ * modulation and SNR in one submission: the host groups the drops by kernel class
+ * (antennas, threads, samples per thread) into per-drop plans, the device runs one
+ * launch per class (capturable into one graph).  Replaces the reference's drop loop
+ * over independently configured simulations (MonteCarloActor.__execute_stages,
+ * hermespy/core/pymonte/actors.py:316-452), which runs one shape at a time. */
+typedef struct hpx_drop_tables {
+  const int32_t* delays;     /* device arrays of hpx_run_batch for this drop's link */
+  const double* path_gains;
+  const hpx_c128* corr_rx;   /* or NULL */
+  const hpx_c128* corr_tx;
+} hpx_drop_tables;
+
+typedef struct hpx_uneven_class {
+  int64_t first, count;      /* contiguous plan slots of the class */
+  int32_t num_tx, threads, pairs, reserved;
+  int64_t lds_bytes;         /* dynamic LDS per workgroup (max over the class) */
+} hpx_uneven_class;
+
+/* Bytes of one drop plan. */
+int64_t hpx_uneven_plan_bytes(void);
+
+/* Builds the plans on the host (no device work): drop i has link links_host[i], tables
+ * tables_host[i], Philox drop index drop_index_host[i], noise power noise_power[i] and error
+ * count errors[i] (device arrays [num_drops]).  plans_host (num_drops plan bytes) receives the
+ * plans grouped by class, heavier classes and drops first; order_host[k] = the drop at slot k;
+ * classes_host (capacity max_classes) the classes.  Copy plans_host to the device for
+ * hpx_uneven_run. */
+int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_host, const uint32_t* drop_index_host,
+                       int64_t num_drops, uint64_t seed, const double* noise_power, int32_t* errors, void* plans_host,
+                       int64_t* order_host, hpx_uneven_class* classes_host, int max_classes, int* num_classes_host);
+
+/* One launch per class from the device copy of the plans. */
+int hpx_uneven_run(const void* plans, const hpx_uneven_class* classes_host, int num_classes, void* stream);
 
 /* Host twin of the per-drop detector hpx_run_batch builds for ideal CSI, where every
  * resource element's channel is H_k = c_k S: S^H S = V diag(lambda) V^H (cyclic

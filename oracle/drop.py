@@ -47,13 +47,26 @@ class Link:
     dc_suppression: bool = True
     gain: float = 1.0
     device_power: float = 1.0
+    frame: ofdm.Frame | None = None  # None: all-data frame of `words` words
+    pilots: np.ndarray | None = None  # (num_tx, num_references) reference symbols of each stream
     params: fading.FadingParameters = field(init=False)
-    frame: ofdm.Frame = field(init=False)
 
     def __post_init__(self) -> None:
         self.params = fading.make_parameters(self.delays, self.power_profile, self.rice_factors, self.num_sinusoids,
                                              self.doppler, None, self.gain, self.corr_rx, self.corr_tx)
-        self.frame = ofdm.simple_frame(self.num_subcarriers, self.words, self.prefix_ratio, self.dc_suppression)
+        if self.frame is None:
+            self.frame = ofdm.simple_frame(self.num_subcarriers, self.words, self.prefix_ratio, self.dc_suppression)
+        if self.frame.num_references and (self.pilots is None or np.shape(self.pilots) !=
+                                          (self.num_tx, self.frame.num_references)):
+            raise ValueError("a frame with reference elements needs (num_tx, num_references) pilot symbols")
+
+    def with_doppler(self, doppler: float) -> "Link":
+        """The same link at another (NLOS and LOS) Doppler."""
+
+        other = Link(**{f: getattr(self, f) for f in self.__dataclass_fields__ if f != "params"})
+        other.params = fading.make_parameters(self.delays, self.power_profile, self.rice_factors, self.num_sinusoids,
+                                              doppler, None, self.gain, self.corr_rx, self.corr_tx)
+        return other
 
     @property
     def bits_per_symbol(self) -> int:
@@ -81,6 +94,8 @@ class DropResult:
     rx_bits: np.ndarray
     equalized: np.ndarray
     received: np.ndarray  # time-domain frame after noise (Nrx, M)
+    grid: np.ndarray | None = None  # (Nrx, words, N) demodulated grid
+    csi: np.ndarray | None = None  # (Nrx, Ntx, words, N) ideal CSI
 
 
 def realize(link: Link, seed: int, drop: int) -> fading.FadingSample:
@@ -92,7 +107,9 @@ def transmit(link: Link, seed: int, drop: int):
     """Bits and the (Ntx, M) transmitted frame."""
 
     bits = np.stack([philox.stream_bits(seed, drop, t, link.bits_per_stream) for t in range(link.num_tx)])
-    x = np.stack([ofdm.modulate(link.frame, ofdm.place(link.frame, mapping.get_symbols(bits[t], link.modulation_order)))
+    refs = (lambda t: None) if link.pilots is None else (lambda t: link.pilots[t])
+    x = np.stack([ofdm.modulate(link.frame, ofdm.place(link.frame, mapping.get_symbols(bits[t], link.modulation_order),
+                                                       refs(t)))
                   for t in range(link.num_tx)])
     return bits, x * link.device_power ** 0.5
 
@@ -118,4 +135,4 @@ def run_drop(link: Link, seed: int, drop: int, snr_db: float, faithful: bool = T
         xh = equalize.mmse(yd, hd, p if noise else 0.0)
     rx_bits = np.stack([mapping.detect_bits(xh[t], link.modulation_order).astype(np.uint8) for t in range(xh.shape[0])])
     errors = int(np.sum(np.abs(tx_bits.astype(np.int8) - rx_bits.astype(np.int8))))
-    return DropResult(errors, int(tx_bits.size), tx_bits, rx_bits, xh, rx_frame)
+    return DropResult(errors, int(tx_bits.size), tx_bits, rx_bits, xh, rx_frame, grid, h)

@@ -17,12 +17,16 @@ def pytest_configure(config):
 def oracle_link(cfg, detector=None):
     """oracle.drop.Link with the parameters of a hermespy_amd LinkConfig."""
 
-    from oracle import drop
+    from oracle import drop, ofdm
 
+    frame = None
+    if cfg.frame_mask is not None:
+        frame = ofdm.Frame(cfg.num_subcarriers, np.full(cfg.words, cfg.prefix, dtype=int), cfg.frame_mask.copy(),
+                           cfg.dc_suppression)
     return drop.Link(cfg.num_rx, cfg.num_tx, cfg.num_subcarriers, cfg.words, cfg.prefix_ratio, cfg.modulation_order,
                      cfg.bandwidth, cfg.delays, cfg.power_profile, cfg.rice_factors, cfg.doppler, cfg.num_sinusoids,
                      cfg.corr_rx, cfg.corr_tx, detector or cfg.detector, cfg.dc_suppression, cfg.gain,
-                     cfg.device_power)
+                     cfg.device_power, frame, cfg.pilots)
 
 
 @pytest.fixture(scope="session")

@@ -290,3 +290,24 @@ def test_drop_faithful_equals_fast():
     b = drop.run_drop(link, 7, 3, 15.0, faithful=False)
     assert rel_err(b.equalized, a.equalized) < 1e-10
     assert a.errors == b.errors
+
+
+def test_orthogonal_comb_pilots_isolate_each_stream():
+    """On a static channel without noise, the received grid at stream t's pilots divided by the pilot is
+    exactly the ideal CSI H[r, t] there: the other streams are silent on those elements."""
+
+    from conftest import oracle_link
+    from hermespy_amd.configs import mimo2x2_512_pilots
+    from oracle import drop as odrop
+    from oracle import ofdm
+
+    cfg = mimo2x2_512_pilots()
+    link = oracle_link(cfg).with_doppler(0.0)
+    res = odrop.run_drop(link, 7, 3, 30.0, faithful=False, noise=False)
+    ref = link.frame.mask == ofdm.REFERENCE
+    for t in range(cfg.num_tx):
+        own = np.zeros(ref.shape, dtype=bool)
+        own[ref] = cfg.pilots[t] != 0
+        assert own.sum() == cfg.num_references // cfg.num_tx
+        ls = res.grid[:, own] / cfg.pilots[t][cfg.pilots[t] != 0]
+        np.testing.assert_allclose(ls, res.csi[:, t][:, own], rtol=0, atol=1e-12)

@@ -1,0 +1,130 @@
+"""Heterogeneous batches: the host planner groups a mixed population into kernel classes
+(no device needed), and on the GPU every drop of a graph-replayed mixed launch reproduces
+the single-link launch of the same drop bit for bit."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+
+def test_population_covers_the_mix():
+    from hermespy_amd.uneven import draw_population
+
+    pop = draw_population(4000, seed=3)
+    ants = {s.antennas for s in pop}
+    ns = {s.num_subcarriers for s in pop}
+    assert ants == {1, 2, 4, 8} and ns == {128, 256, 512, 1024, 2048}
+    assert not any(s.antennas == 8 and s.num_subcarriers == 2048 for s in pop)
+    assert {s.modulation_order for s in pop} == {4, 16, 64}
+    paths = np.array([s.num_paths for s in pop])
+    assert paths.min() == 5 and paths.max() == 23
+    dop = np.array([s.doppler_hz for s in pop])
+    assert 10 <= dop.min() and dop.max() <= 900
+
+
+def test_planner_groups_by_kernel_class(native_lib):
+    """hpx_uneven_prepare on the host: classes are contiguous runs of one (antennas, threads, pairs),
+    heavier classes first, every drop placed once, LDS = max over the class."""
+
+    from hermespy_amd import _lib
+    from hermespy_amd.uneven import MAX_CLASSES, _c_link, draw_population
+
+    pop = draw_population(300, seed=11)
+    links = [s.link() for s in pop]
+    n = len(links)
+    c_links = (_lib.HpxLink * n)(*[_c_link(c) for c in links])
+    tables = (_lib.HpxDropTables * n)()
+    for i in range(n):
+        tables[i].delays = 0x1000 + 256 * i  # host planning never dereferences device pointers
+        tables[i].path_gains = 0x100000 + 512 * i
+    idx = np.arange(n, dtype=np.uint32) + 17
+    plan_bytes = int(native_lib.hpx_uneven_plan_bytes())
+    plans = np.zeros(n * plan_bytes, dtype=np.uint8)
+    order = np.empty(n, dtype=np.int64)
+    classes = (_lib.HpxUnevenClass * MAX_CLASSES)()
+    nc = _lib.c_int(0)
+    _lib.check(native_lib.hpx_uneven_prepare(c_links, tables, idx.ctypes.data, n, ctypes.c_uint64(5), 0x2000, 0x3000,
+                                             plans.ctypes.data, order.ctypes.data, classes, MAX_CLASSES,
+                                             ctypes.byref(nc)))
+    assert sorted(order.tolist()) == list(range(n))
+    seen = 0
+    for k in range(nc.value):
+        c = classes[k]
+        assert c.first == seen and c.count > 0
+        members = order[c.first:c.first + c.count]
+        shapes = {(links[i].num_tx, links[i].num_subcarriers) for i in members}
+        assert len(shapes) == 1 and next(iter(shapes))[0] == c.num_tx
+        lds = [int(native_lib.hpx_run_batch_lds_bytes(ctypes.byref(c_links[i]))) for i in members]
+        assert c.lds_bytes == max(lds)
+        assert c.threads * c.pairs * 2 == links[members[0]].num_subcarriers
+        seen += c.count
+    assert seen == n
+
+
+def test_planner_rejects_bad_links(native_lib):
+    from hermespy_amd import _lib
+    from hermespy_amd.uneven import MAX_CLASSES, DropShape, _c_link
+
+    bad = DropShape(3, 256, 5, 2.0, 100.0, 16, 10.0).link()  # 3x3 is not instantiated
+    c_links = (_lib.HpxLink * 1)(_c_link(bad))
+    tables = (_lib.HpxDropTables * 1)()
+    tables[0].delays, tables[0].path_gains = 0x1000, 0x2000
+    idx = np.zeros(1, dtype=np.uint32)
+    plans = np.zeros(int(native_lib.hpx_uneven_plan_bytes()), dtype=np.uint8)
+    order = np.empty(1, dtype=np.int64)
+    classes = (_lib.HpxUnevenClass * MAX_CLASSES)()
+    nc = _lib.c_int(0)
+    st = native_lib.hpx_uneven_prepare(c_links, tables, idx.ctypes.data, 1, ctypes.c_uint64(0), 0x10, 0x20,
+                                       plans.ctypes.data, order.ctypes.data, classes, MAX_CLASSES, ctypes.byref(nc))
+    with pytest.raises(ValueError, match="not instantiated"):
+        _lib.check(st)
+
+
+@pytest.mark.gpu
+def test_uneven_graph_replay_matches_single_link_launches(cuda):
+    import torch
+
+    from hermespy_amd.simulation import BatchedLink
+    from hermespy_amd.uneven import UnevenBatch, draw_population
+
+    pop = draw_population(96, seed=21)
+    ub = UnevenBatch(pop, seed=99, first_drop=1000)
+    ub.run()
+    torch.cuda.synchronize()
+    eager = ub.errors.cpu().numpy().copy()
+    ub.capture()
+    for rep in range(3):
+        ub.errors.fill_(-1)
+        torch.cuda.synchronize()
+        ub.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(ub.errors.cpu().numpy(), eager), rep
+    # each drop equals the plain single-link launch of the same drop
+    for i in range(0, len(pop), 7):
+        cfg = pop[i].link()
+        bl = BatchedLink(cfg)
+        npw = torch.as_tensor([cfg.noise_power(pop[i].snr_db)], dtype=torch.float64, device=cuda)
+        err = torch.empty(1, dtype=torch.int32, device=cuda)
+        bl.run(99, 1000 + i, npw, err)
+        torch.cuda.synchronize()
+        assert int(err.item()) == int(eager[i]), (i, pop[i])
+    assert 0 < eager.sum()
+
+
+@pytest.mark.gpu
+def test_uneven_autotuner_keeps_results(cuda):
+    import torch
+
+    from hermespy_amd.uneven import UnevenBatch, draw_population, tune
+
+    pop = draw_population(64, seed=5)
+    ub = UnevenBatch(pop, seed=1)
+    ub.run()
+    torch.cuda.synchronize()
+    before = ub.errors.cpu().numpy().copy()
+    info = tune(ub, reps=1)
+    ub.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(ub.errors.cpu().numpy(), before)
+    assert info["choice"]
