@@ -245,7 +245,8 @@ class Trainer:
 
 
 def train(cfg: LinkConfig, num_drops: int, drops_per_step: int = 1024, lr: float = 2e-3, log_every: int = 50,
-          first_drop: int = 1 << 30, params: list | None = None, verbose: bool = True) -> ConvNetEstimator:
+          first_drop: int = 1 << 30, params: list | None = None, verbose: bool = True,
+          save_to: str | Path | None = None, save_every: int = 100) -> ConvNetEstimator:
     """Train on num_drops simulator drops (drop indices from first_drop, disjoint from evaluation drops);
     cosine learning-rate decay."""
 
@@ -259,6 +260,8 @@ def train(cfg: LinkConfig, num_drops: int, drops_per_step: int = 1024, lr: float
         if verbose and (k % log_every == 0 or k == steps - 1):
             print(f"step {k + 1}/{steps} nmse {10 * math.log10(max(float(loss), 1e-30)):.2f} dB "
                   f"lr {rate:.2e} {time.perf_counter() - t0:.1f} s", flush=True)
+        if save_to is not None and (k + 1) % save_every == 0:
+            net.save(save_to)
     return net
 
 
@@ -273,9 +276,9 @@ def _main() -> None:
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", type=str, default=str(WEIGHTS_PATH))
     args = ap.parse_args()
-    net = train(mimo2x2_512_pilots(), args.drops, args.batch, args.lr)
-    torch.cuda.synchronize()
     Path(args.out).parent.mkdir(parents=True, exist_ok=True)
+    net = train(mimo2x2_512_pilots(), args.drops, args.batch, args.lr, save_to=args.out)
+    torch.cuda.synchronize()
     net.save(args.out)
     print(f"saved {net.num_params} parameters to {args.out}", flush=True)
 
