@@ -156,9 +156,9 @@ def estimator_leg(drops: int = 4096, reps: int = 10) -> dict:
                     "dtype": "f32", **quality(lm.apply(ls, sidx, out, rows))}
     if WEIGHTS_PATH.exists():
         net = ConvNetEstimator.load(cfg)
-        res["learned"] = {"drops_per_s": timed(lambda: net.estimate(lm.pilot_ls(grid, drops), out)),
+        res["learned"] = {"drops_per_s": timed(lambda: net.estimate_grid(grid, lm.pilot_index, lm.pilot_values, out)),
                           "dtype": "bf16 weights and activations, f32 accumulate", "params": net.num_params,
-                          **quality(net.estimate(ls, out))}
+                          **quality(net.estimate_grid(grid, lm.pilot_index, lm.pilot_values, out))}
         res["throughput_ratio"] = res["learned"]["drops_per_s"] / res["lmmse"]["drops_per_s"]
         gap = np.asarray(res["learned"]["nmse_db"]) - np.asarray(res["lmmse"]["nmse_db"])
         res["nmse_gap_db"] = gap.tolist()
@@ -195,6 +195,35 @@ def uneven_leg(drops: int = 8192, reps: int = 5) -> dict:
             "workgroup_choice": tuning["choice"]}
 
 
+def massive_leg(drops: int = 64, chunk: int = 32) -> dict:
+    """64x8 massive-MIMO uplink drops (4096-point OFDM, 3276 active subcarriers, 256-QAM, LMMSE)."""
+
+    import torch
+
+    from hermespy_amd.massive import MassiveBatch, MassiveConfig
+
+    cfg = MassiveConfig()
+    mb = MassiveBatch(cfg, chunk=chunk)
+    snr = np.asarray(cfg.snr_db)
+    err = torch.zeros(drops, dtype=torch.int32, device="cuda")
+    npw = torch.as_tensor(cfg.noise_power(snr[np.arange(drops) % snr.size]), dtype=torch.float64, device="cuda")
+    mb.run(SEED, 0, npw[:chunk], err[:chunk])
+    torch.cuda.synchronize()
+    err.zero_()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for s0 in range(0, drops, chunk):
+        mb.run(SEED, s0, npw[s0:s0 + chunk], err[s0:s0 + chunk])
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b)
+    e = err.cpu().numpy()
+    idx = np.arange(drops) % snr.size
+    ber = [float(e[idx == k].sum() / ((idx == k).sum() * cfg.bits_per_drop)) for k in range(snr.size)]
+    return {"workload": cfg.name, "drops": drops, "ms": ms, "drops_per_s": drops / (ms / 1e3),
+            "bits_per_s": drops * cfg.bits_per_drop / (ms / 1e3), "snr_db": snr.tolist(), "ber": ber}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +236,7 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="use the default workgroup size instead of the autotuner's")
     ap.add_argument("--no-estimators", action="store_true", help="skip the pilot-estimation leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the heterogeneous-batch leg")
+    ap.add_argument("--no-massive", action="store_true", help="skip the massive-MIMO leg")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (counter calibration for profiles)")
     args = ap.parse_args()
@@ -294,6 +324,8 @@ def main() -> int:
         out["estimators"] = estimator_leg()
     if not args.no_uneven:
         out["uneven_batch"] = uneven_leg()
+    if not args.no_massive:
+        out["massive_mimo"] = massive_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -84,6 +84,21 @@ class HpxUnevenClass(ctypes.Structure):
                 ("reserved", c_i32), ("lds_bytes", c_i64)]
 
 
+class HpxMassiveLink(ctypes.Structure):
+    """Mirror of ``hpx_massive_link``."""
+
+    _fields_ = [("num_rx", c_i32), ("num_users", c_i32), ("num_subcarriers", c_i32), ("active", c_i32),
+                ("words", c_i32), ("prefix", c_i32), ("modulation_order", c_i32), ("num_paths", c_i32),
+                ("num_sinusoids", c_i32), ("max_delay", c_i32), ("dc_suppression", c_i32), ("bandwidth", c_dbl),
+                ("doppler", c_dbl), ("los_doppler", c_dbl), ("gain", c_dbl), ("device_power", c_dbl)]
+
+
+class HpxMassiveWork(ctypes.Structure):
+    """Mirror of ``hpx_massive_work`` (device pointers)."""
+
+    _fields_ = [(name, c_vp) for name in ("spatial", "coef", "abar", "det_v", "det_a", "det_l", "z", "grid", "csi")]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "hpx_abi_version": (c_int, []),
@@ -126,10 +141,16 @@ SIGNATURES = {
     "hpx_net_input": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp]),
     "hpx_net_output": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp]),
     "hpx_estimate_error": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_i64, c_vp, c_vp, c_vp]),
-    "hpx_nmse_grad": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "hpx_nmse_grad": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_int, c_vp, c_vp, c_vp,
+                              c_vp]),
     "hpx_adam_step": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                               ctypes.c_float, c_i64, c_vp]),
     "hpx_conv1d_weight_copies": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
+    "hpx_learned_estimate": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_int,
+                                     c_vp, c_vp]),
+    "hpx_massive_sizes": (c_int, [ctypes.POINTER(HpxMassiveLink), c_vp]),
+    "hpx_massive_run": (c_int, [ctypes.POINTER(HpxMassiveLink), c_vp, c_vp, c_vp, c_vp, c_u64, c_i64, c_i64, c_vp,
+                                ctypes.POINTER(HpxMassiveWork), c_vp, c_vp, c_vp]),
     "hpx_ls_estimate": (c_int, [c_vp, c_int, c_i64, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_vp, c_vp,
                                 c_vp, c_vp]),
 }

@@ -52,7 +52,7 @@ __device__ __forceinline__ float gelu_grad(float x) {
 // out[s][p][co] = bias[co] + sum_{k, ci} w[co][k][ci] in[s][p + k - 2][ci]
 // One workgroup per sample; wave v computes rows 32 (v & 1) .. +32 of output column
 // tiles v / 2, v / 2 + 2, ...  mode 0: out_f32 = acc (+ bias), out_bf16 = bf16(that);
-// mode 1: out_f32 = pre-activation, out_bf16 = bf16(gelu(pre)).
+// mode 1: out_f32 = pre-activation, out_bf16 = bf16(gelu(pre)); mode 2: out_f32 += acc (+ bias).
 template <int CIN>
 __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t* __restrict__ in,
                                                        const uint16_t* __restrict__ w,
@@ -94,6 +94,10 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t* __restric
       const int p = 32 * half + (r >> 2) * 8 + kq * 4 + (r & 3);
       const int64_t o = (static_cast<int64_t>(s) * kPos + p) * cout + co;
       const float v = acc[r] + bv;
+      if (mode == 2) {
+        out_f32[o] += v;
+        continue;
+      }
       if (out_f32) out_f32[o] = v;
       if (out_bf16) out_bf16[o] = f2bf(mode == 1 ? gelu(v) : v);
     }
@@ -263,54 +267,88 @@ __global__ void __launch_bounds__(256) estimate_error_kernel(const float2* __res
   }
 }
 
-// loss gradient of the batch NMSE sum |out - H|^2 / sum |H|^2 wrt the network output (the
-// normaliser sum |H|^2 is computed beforehand into norm[0])
-__global__ void __launch_bounds__(256) nmse_grad_kernel(const float* __restrict__ out, const float2* __restrict__ csi,
+// Loss of a batch: mean over SNR groups of the NMSE in dB, 10 log10(sum |out - H|^2 / sum |H|^2) per
+// group, so that every SNR point weighs alike (a batch NMSE would be dominated by the noisiest
+// drops).  Pass 1 accumulates per-group error and power (one workgroup per sample), pass 2 turns
+// them into the loss and per-group gradient scales, pass 3 writes d loss / d out.
+__device__ __forceinline__ int64_t out_index(int64_t s, int e, int words, int n) {
+  const int w = e / n, q = e - w * n;
+  const int blk = n / kPos;
+  const int p = q / blk, u = q - p * blk;
+  return (s * kPos + p) * (2 * words * blk) + 2 * (w * blk + u);
+}
+
+__device__ __forceinline__ int sample_group(int64_t s, int num_drops, int num_rx, const int32_t* group) {
+  const int64_t d = (s % (static_cast<int64_t>(num_drops) * num_rx)) / num_rx;
+  return group ? group[d] : 0;
+}
+
+__global__ void __launch_bounds__(256) loss_accumulate_kernel(const float* __restrict__ out,
+                                                              const float2* __restrict__ csi,
+                                                              const float2* __restrict__ spatial, int num_drops,
+                                                              int num_rx, int num_tx, int words, int n,
+                                                              const int32_t* __restrict__ group, int num_groups,
+                                                              double* __restrict__ work) {
+  __shared__ double red[2][4];
+  const int64_t s = blockIdx.x;
+  const int64_t wn = static_cast<int64_t>(words) * n;
+  double e2 = 0.0, h2 = 0.0;
+  for (int e = threadIdx.x; e < wn; e += blockDim.x) {
+    const float2 h = ideal_h(csi, spatial, s, e, num_drops, num_rx, num_tx, wn);
+    const int64_t o = out_index(s, e, words, n);
+    const float dr = out[o] - h.x, di = out[o + 1] - h.y;
+    e2 += static_cast<double>(dr * dr + di * di);
+    h2 += static_cast<double>(cabs2(h));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    e2 += __shfl_down(e2, off, 64);
+    h2 += __shfl_down(h2, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = e2;
+    red[1][threadIdx.x >> 6] = h2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int g = sample_group(s, num_drops, num_rx, group);
+    atomicAdd(work + g, red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+    atomicAdd(work + num_groups + g, red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+  }
+}
+
+__global__ void loss_finish_kernel(int num_groups, double* __restrict__ work, double* __restrict__ loss) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double l = 0.0;
+  int used = 0;
+  for (int g = 0; g < num_groups; ++g) {
+    const double err = work[g], pw = work[num_groups + g];
+    if (err > 0.0 && pw > 0.0) {
+      l += 10.0 * log10(err / pw);
+      ++used;
+    }
+  }
+  // d/d out of mean_g 10 log10(err_g / pow_g) = (10 / ln 10) / (used err_g) * 2 (out - H)
+  for (int g = 0; g < num_groups; ++g)
+    work[2 * num_groups + g] = (work[g] > 0.0 && used) ? 2.0 * 4.342944819032518 / (used * work[g]) : 0.0;
+  loss[0] = used ? l / used : 0.0;
+}
+
+__global__ void __launch_bounds__(256) loss_grad_kernel(const float* __restrict__ out, const float2* __restrict__ csi,
                                                         const float2* __restrict__ spatial, int num_drops, int num_rx,
-                                                        int num_tx, int words, int n, const double* __restrict__ norm,
-                                                        float* __restrict__ grad, double* __restrict__ loss) {
-  __shared__ double red[4];
+                                                        int num_tx, int words, int n, const int32_t* __restrict__ group,
+                                                        int num_groups, const double* __restrict__ work,
+                                                        float* __restrict__ grad) {
   const int64_t wn = static_cast<int64_t>(words) * n;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t num_samples = static_cast<int64_t>(num_tx) * num_drops * num_rx;
-  double l = 0.0;
-  if (i < num_samples * wn) {
-    const int64_t s = i / wn;
-    const int e = static_cast<int>(i - s * wn);
-    const int w = e / n, q = e - w * n;
-    const int blk = n / kPos;
-    const int p = q / blk, u = q - p * blk;
-    const int cout = 2 * words * blk;
-    const int64_t o = (s * kPos + p) * cout + 2 * (w * blk + u);
-    const float2 h = ideal_h(csi, spatial, s, e, num_drops, num_rx, num_tx, wn);
-    const float dr = out[o] - h.x, di = out[o + 1] - h.y;
-    const float scale = static_cast<float>(2.0 / norm[0]);
-    grad[o] = scale * dr;
-    grad[o + 1] = scale * di;
-    l = static_cast<double>(dr * dr + di * di) / norm[0];
-  }
-  for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
-  __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(loss, red[0] + red[1] + red[2] + red[3]);
-}
-
-__global__ void __launch_bounds__(256) channel_power_kernel(const float2* __restrict__ csi,
-                                                            const float2* __restrict__ spatial, int num_drops,
-                                                            int num_rx, int num_tx, int64_t wn,
-                                                            double* __restrict__ norm) {
-  __shared__ double red[4];
-  const int64_t total = static_cast<int64_t>(num_tx) * num_drops * num_rx * wn;
-  double acc = 0.0;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
-       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int64_t s = i / wn;
-    acc += static_cast<double>(cabs2(ideal_h(csi, spatial, s, i - s * wn, num_drops, num_rx, num_tx, wn)));
-  }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(norm, red[0] + red[1] + red[2] + red[3]);
+  if (i >= num_samples * wn) return;
+  const int64_t s = i / wn;
+  const int e = static_cast<int>(i - s * wn);
+  const float2 h = ideal_h(csi, spatial, s, e, num_drops, num_rx, num_tx, wn);
+  const int64_t o = out_index(s, e, words, n);
+  const float scale = static_cast<float>(work[2 * num_groups + sample_group(s, num_drops, num_rx, group)]);
+  grad[o] = scale * (out[o] - h.x);
+  grad[o + 1] = scale * (out[o + 1] - h.y);
 }
 
 // ---------------------------------------------------------------- Adam + weight copies
@@ -343,6 +381,147 @@ __global__ void __launch_bounds__(256) weight_copies_kernel(const float* __restr
   if (wt) wt[(static_cast<int64_t>(ci) * kTaps + (kTaps - 1 - k)) * cout + co] = v;
 }
 
+// ---------------------------------------------------------------- fused inference
+// The whole estimator for G samples per workgroup: LS at the sample's pilots (y_p / x_p), the four
+// layers and the skip convolution, with every activation in LDS (bfloat16, zero halo rows), and
+// the output written straight into the estimate grid.  Numerically the layer-by-layer path: the
+// same float32 accumulation, bias, GELU and bfloat16 rounding between layers.
+constexpr int kFusedSamples = 2;
+constexpr int kS8 = 8 + 4, kS64 = 64 + 4;  // padded LDS rows (8-byte reads conflict-free per half wave)
+
+template <int CIN, int STRIDE_IN>
+__device__ __forceinline__ void fused_tile(const uint16_t (*in)[(kPos + 2 * kHalo) * STRIDE_IN],
+                                           const uint16_t* __restrict__ w, int co, int half, int m, int kq,
+                                           floatx16 (&acc)[kFusedSamples]) {
+  const uint16_t* wrow = w + static_cast<int64_t>(co) * kTaps * CIN + 4 * kq;
+#pragma unroll
+  for (int k = 0; k < kTaps; ++k) {
+#pragma unroll 2
+    for (int c0 = 0; c0 < CIN; c0 += 8) {
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(wrow + k * CIN + c0);
+#pragma unroll
+      for (int g = 0; g < kFusedSamples; ++g) {
+        const bf16x4 a =
+            *reinterpret_cast<const bf16x4*>(&in[g][(32 * half + m + k) * STRIDE_IN + 4 * kq + c0]);
+        acc[g] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[g], 0, 0, 0);
+      }
+    }
+  }
+}
+
+template <int CIN, int STRIDE_IN>
+__device__ __forceinline__ void fused_hidden(const uint16_t (*in)[(kPos + 2 * kHalo) * STRIDE_IN],
+                                             uint16_t (*out)[(kPos + 2 * kHalo) * kS64],
+                                             const uint16_t* __restrict__ w, const float* __restrict__ bias) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int half = wave & 1, ct = wave >> 1;  // 64 outputs = 2 column tiles x 2 row halves
+  const int m = lane & 31, kq = lane >> 5;
+  const int co = ct * 32 + m;
+  floatx16 acc[kFusedSamples];
+#pragma unroll
+  for (int g = 0; g < kFusedSamples; ++g)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[g][r] = 0.0f;
+  fused_tile<CIN, STRIDE_IN>(in, w, co, half, m, kq, acc);
+  const float bv = bias[co];
+#pragma unroll
+  for (int g = 0; g < kFusedSamples; ++g)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int p = 32 * half + (r >> 2) * 8 + kq * 4 + (r & 3);
+      out[g][(p + kHalo) * kS64 + co] = f2bf(gelu(acc[g][r] + bv));
+    }
+}
+
+__global__ void __launch_bounds__(256) learned_estimate_kernel(
+    const float2* __restrict__ grid, int num_drops, int num_rx, int words, int n, const int32_t* __restrict__ pidx,
+    const float2* __restrict__ pval, int num_pilots, int64_t num_samples, const uint16_t* __restrict__ w1,
+    const float* __restrict__ b1, const uint16_t* __restrict__ w2, const float* __restrict__ b2,
+    const uint16_t* __restrict__ w3, const float* __restrict__ b3, const uint16_t* __restrict__ w4,
+    const float* __restrict__ b4, const uint16_t* __restrict__ wsk, const float* __restrict__ bsk, int cout,
+    float* __restrict__ est) {
+  __shared__ __align__(16) uint16_t xs[kFusedSamples][(kPos + 2 * kHalo) * kS8];
+  __shared__ __align__(16) uint16_t a0[kFusedSamples][(kPos + 2 * kHalo) * kS64];
+  __shared__ __align__(16) uint16_t a1[kFusedSamples][(kPos + 2 * kHalo) * kS64];
+  const int tid = threadIdx.x;
+  const int64_t s0 = static_cast<int64_t>(blockIdx.x) * kFusedSamples;
+  const int64_t wn = static_cast<int64_t>(words) * n;
+  const int64_t dr_count = static_cast<int64_t>(num_drops) * num_rx;
+  const int pilot_words = num_pilots / kPos;
+  // zero halo rows of every buffer
+  for (int i = tid; i < kFusedSamples * 2 * kHalo * kS64; i += 256) {
+    const int g = i / (2 * kHalo * kS64), rest = i - g * 2 * kHalo * kS64;
+    const int row = rest / kS64, c = rest - row * kS64;
+    const int rr = row < kHalo ? row : kPos + row;  // rows 0, 1 and 66, 67
+    a0[g][rr * kS64 + c] = 0;
+    a1[g][rr * kS64 + c] = 0;
+    if (c < kS8) xs[g][rr * kS8 + c] = 0;
+  }
+  // input: LS at the sample's own pilots, two channels per pilot word, then the stream flag
+  for (int i = tid; i < kFusedSamples * kPos; i += 256) {
+    const int g = i / kPos, p = i - g * kPos;
+    const int64_t s = s0 + g;
+    uint16_t* row = &xs[g][(p + kHalo) * kS8];
+    if (s >= num_samples) {
+      for (int c = 0; c < 8; ++c) row[c] = 0;
+      continue;
+    }
+    const int t = static_cast<int>(s / dr_count);
+    const int64_t dr = s - t * dr_count;
+    for (int v = 0; v < pilot_words; ++v) {
+      const int j = v * kPos + p;
+      const float2 x = pval[t * num_pilots + j];
+      const float2 y = grid[dr * wn + pidx[t * num_pilots + j]];
+      const float2 h = cscale(cmulc(x, y), 1.0f / cabs2(x));
+      row[2 * v] = f2bf(h.x);
+      row[2 * v + 1] = f2bf(h.y);
+    }
+    row[2 * pilot_words] = f2bf(t ? 1.0f : -1.0f);
+    for (int c = 2 * pilot_words + 1; c < 8; ++c) row[c] = 0;
+  }
+  __syncthreads();
+  fused_hidden<8, kS8>(xs, a0, w1, b1);
+  __syncthreads();
+  fused_hidden<64, kS64>(a0, a1, w2, b2);
+  __syncthreads();
+  fused_hidden<64, kS64>(a1, a0, w3, b3);
+  __syncthreads();
+  // last layer + skip convolution straight into the estimate grid:
+  // channel co = 2 (w B + u) + {re, im} of position p -> est[s][w][B p + u]
+  const int wave = tid >> 6, lane = tid & 63;
+  const int half = wave & 1;
+  const int m = lane & 31, kq = lane >> 5;
+  const int blk = n / kPos;
+  for (int ct = wave >> 1; ct < cout / 32; ct += 2) {
+    const int co = ct * 32 + m;
+    floatx16 acc[kFusedSamples];
+#pragma unroll
+    for (int g = 0; g < kFusedSamples; ++g)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[g][r] = 0.0f;
+    fused_tile<64, kS64>(a0, w4, co, half, m, kq, acc);
+    floatx16 sk[kFusedSamples];
+#pragma unroll
+    for (int g = 0; g < kFusedSamples; ++g)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) sk[g][r] = 0.0f;
+    fused_tile<8, kS8>(xs, wsk, co, half, m, kq, sk);
+    const float bv = b4[co], bs = bsk[co];
+    const int c2 = co >> 1, reim = co & 1;
+    const int w = c2 / blk, u = c2 - w * blk;
+#pragma unroll
+    for (int g = 0; g < kFusedSamples; ++g) {
+      const int64_t s = s0 + g;
+      if (s >= num_samples) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int p = 32 * half + (r >> 2) * 8 + kq * 4 + (r & 3);
+        est[2 * (s * wn + static_cast<int64_t>(w) * n + blk * p + u) + reim] = (acc[g][r] + bv) + (sk[g][r] + bs);
+      }
+    }
+  }
+}
+
 unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
 }  // namespace
@@ -352,9 +531,9 @@ extern "C" int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int c
                                   void* stream) {
   HPX_REQUIRE(num_samples >= 0 && num_samples <= 0x7fffffff, "hpx_conv1d_forward: invalid sample count");
   HPX_REQUIRE(cout >= 32 && cout % 32 == 0, "hpx_conv1d_forward: cout must be a multiple of 32 (got %d)", cout);
-  HPX_REQUIRE(mode == 0 || mode == 1, "hpx_conv1d_forward: mode must be 0 (linear) or 1 (GELU)");
+  HPX_REQUIRE(mode >= 0 && mode <= 2, "hpx_conv1d_forward: mode must be 0 (linear), 1 (GELU) or 2 (accumulate)");
   if (num_samples == 0) return HPX_OK;
-  HPX_REQUIRE(in && weight && (out_f32 || out_bf16), "hpx_conv1d_forward: null pointer");
+  HPX_REQUIRE(in && weight && (out_f32 || out_bf16) && (mode != 2 || out_f32), "hpx_conv1d_forward: null pointer");
   const dim3 g(static_cast<unsigned>(num_samples));
   const hipStream_t s = as_stream(stream);
   switch (cin) {
@@ -452,22 +631,25 @@ extern "C" int hpx_estimate_error(const hpx_c64* est, const hpx_c64* csi, const 
 }
 
 extern "C" int hpx_nmse_grad(const float* out, const hpx_c64* csi, const hpx_c64* spatial, int64_t num_drops,
-                             int num_rx, int num_tx, int words, int num_subcarriers, double* norm, float* grad,
-                             double* loss, void* stream) {
+                             int num_rx, int num_tx, int words, int num_subcarriers, const int32_t* group,
+                             int num_groups, double* work, float* grad, double* loss, void* stream) {
   HPX_REQUIRE(num_drops >= 1 && num_drops <= 0x7fffffff && num_rx >= 1 && num_tx >= 1 && words >= 1 &&
-                  num_subcarriers % kPos == 0,
+                  num_subcarriers % kPos == 0 && num_groups >= 1 && (group || num_groups == 1),
               "hpx_nmse_grad: invalid sizes");
-  HPX_REQUIRE(out && csi && spatial && norm && grad && loss, "hpx_nmse_grad: null pointer");
+  HPX_REQUIRE(out && csi && spatial && work && grad && loss, "hpx_nmse_grad: null pointer");
   const int64_t wn = static_cast<int64_t>(words) * num_subcarriers;
-  const int64_t n = static_cast<int64_t>(num_tx) * num_drops * num_rx * wn;
+  const int64_t samples = static_cast<int64_t>(num_tx) * num_drops * num_rx;
+  HPX_REQUIRE(samples <= 0x7fffffff, "hpx_nmse_grad: too many samples");
   const hipStream_t s = as_stream(stream);
   const auto* c = reinterpret_cast<const float2*>(csi);
   const auto* sp = reinterpret_cast<const float2*>(spatial);
-  // norm[0] and loss[0] must be zero on entry
-  hipLaunchKernelGGL(channel_power_kernel, dim3(static_cast<unsigned>(std::min<int64_t>(blocks_for(n), 2048))),
-                     dim3(256), 0, s, c, sp, static_cast<int>(num_drops), num_rx, num_tx, wn, norm);
-  hipLaunchKernelGGL(nmse_grad_kernel, dim3(blocks_for(n)), dim3(256), 0, s, out, c, sp, static_cast<int>(num_drops),
-                     num_rx, num_tx, words, num_subcarriers, norm, grad, loss);
+  const int nd = static_cast<int>(num_drops);
+  // work[0 .. 2 num_groups) must be zero on entry
+  hipLaunchKernelGGL(loss_accumulate_kernel, dim3(static_cast<unsigned>(samples)), dim3(256), 0, s, out, c, sp, nd,
+                     num_rx, num_tx, words, num_subcarriers, group, num_groups, work);
+  hipLaunchKernelGGL(loss_finish_kernel, dim3(1), dim3(64), 0, s, num_groups, work, loss);
+  hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for(samples * wn)), dim3(256), 0, s, out, c, sp, nd, num_rx,
+                     num_tx, words, num_subcarriers, group, num_groups, work, grad);
   return check_launch("hpx_nmse_grad");
 }
 
@@ -491,4 +673,29 @@ extern "C" int hpx_conv1d_weight_copies(const float* param, int cout, int cin, u
   hipLaunchKernelGGL(weight_copies_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), param, cout, cin,
                      weight, weight_transposed);
   return check_launch("hpx_conv1d_weight_copies");
+}
+
+extern "C" int hpx_learned_estimate(const hpx_c64* grid, int64_t num_drops, int num_rx, int num_tx, int words,
+                                    int num_subcarriers, const int32_t* pilot_index, const hpx_c64* pilot_values,
+                                    int num_pilots, const uint16_t* const* weights, const float* const* biases,
+                                    int cout, hpx_c64* est, void* stream) {
+  HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff && num_rx >= 1 && num_tx >= 1 && words >= 1,
+              "hpx_learned_estimate: invalid sizes");
+  HPX_REQUIRE(num_subcarriers >= kPos && num_subcarriers % kPos == 0 && num_pilots >= kPos && num_pilots % kPos == 0 &&
+                  2 * (num_pilots / kPos) + 1 <= 8,
+              "hpx_learned_estimate: needs 64-position pilot words (at most 3) and a multiple of 64 subcarriers");
+  HPX_REQUIRE(cout == 2 * words * (num_subcarriers / kPos) && cout % 32 == 0,
+              "hpx_learned_estimate: output channels %d do not match the grid", cout);
+  const int64_t samples = static_cast<int64_t>(num_tx) * num_drops * num_rx;
+  if (samples == 0) return HPX_OK;
+  HPX_REQUIRE(grid && pilot_index && pilot_values && weights && biases && est, "hpx_learned_estimate: null pointer");
+  for (int i = 0; i < 5; ++i) HPX_REQUIRE(weights[i] && biases[i], "hpx_learned_estimate: null layer %d", i);
+  const int64_t blocks = (samples + kFusedSamples - 1) / kFusedSamples;
+  HPX_REQUIRE(blocks <= 0x7fffffff, "hpx_learned_estimate: too many samples");
+  hipLaunchKernelGGL(learned_estimate_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, as_stream(stream),
+                     reinterpret_cast<const float2*>(grid), static_cast<int>(num_drops), num_rx, words,
+                     num_subcarriers, pilot_index, reinterpret_cast<const float2*>(pilot_values), num_pilots, samples,
+                     weights[0], biases[0], weights[1], biases[1], weights[2], biases[2], weights[3], biases[3],
+                     weights[4], biases[4], cout, reinterpret_cast<float*>(est));
+  return check_launch("hpx_learned_estimate");
 }

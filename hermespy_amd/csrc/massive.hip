@@ -1,0 +1,667 @@
+// Massive-MIMO uplink drops: many receive antennas (up to 64), single-antenna users
+// (up to 8), large OFDM grids with a centred active band.  One drop does not fit a
+// CU, so the drop runs as four kernels over caller-owned per-drop buffers:
+//
+//   K0 prologue  (workgroup per drop)       realisation (Philox -> Phi -> S = R_rx exp(2 pi j U) R_tx,
+//                                           SoS table), Taylor coefficients of the taps, per-word tap
+//                                           means, and the per-drop detector V, A = V^H S^H, lambda of
+//                                           S^H S (fading.py:468-515, channel_estimation.py:215-295)
+//   K1 transmit  (workgroup per drop, user) bits -> 256-QAM -> OFDM + CP -> the user's tapped delay line
+//                                           over every word's data window (modem.py:797-888,
+//                                           ofdm.py:97-130, fading.py:371-406)
+//   K2 receive   (workgroup per drop, word, antenna group)
+//                                           y_r = sum_t S[r][t] z_t + AWGN, FFT, active bins; group 0
+//                                           also the ideal-CSI factor c of the word (ofdm.py:132-161)
+//   K3 detect    (workgroup per drop, word, 64 elements)
+//                                           LMMSE through the per-drop eigen form: u = A y on the MFMA
+//                                           units (bfloat16 hi/lo split, float32 accumulation), then
+//                                           x = V diag(conj(c) / (|c|^2 lambda + s2)) u, exact 3GPP hard
+//                                           decisions and bit errors (waveform.py:235-250,
+//                                           psk_qam_mapping.py:171-259, evaluators.py:239-258)
+//
+// The reference caps arrays at 10 antennas (its spatial response slices one 10 x 10 block of
+// uniforms, fading.py:476-489); here the block is (max(10, Nrx), max(10, Ntx)), which keeps the
+// reference's draw layout for links of at most 10 antennas.
+#include <math.h>
+
+#include <algorithm>
+
+#include "hpx_common.hpp"
+#include "hpx_eig.hpp"
+#include "hpx_fading.hpp"
+#include "hpx_fft.hpp"
+#include "hpx_mapping.hpp"
+
+namespace hpx {
+int taylor_log2_block(double max_rate, int terms, int max_log2);
+int ilog2_exact(int n);
+}  // namespace hpx
+
+using namespace hpx;
+
+namespace {
+
+constexpr int kMaxRx = 64, kMaxUsers = 8;
+constexpr int kTxThreads = 512, kRxThreads = 512;
+constexpr int kRxGroup = 8;      // antennas per receive workgroup
+constexpr int kRxRows = 2;       // antennas transformed together
+constexpr int kDetElements = 64;   // resource elements per detection workgroup (32 per wave)
+constexpr int kDetThreads = 2 * kDetElements;
+constexpr int kTail = 4;         // registers per thread holding the previous word's tail
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+struct MassiveArgs {
+  int nr, nt, n, log2n, cp, words, active, active_lo, bps, order, dc;
+  Modulation mod;
+  int paths, sinusoids, terms, nnorm, blk_cols;
+  int log2b, nb, maxd, dx, cpl, x0, xs, keep_below, bits_words;
+  int64_t frame_len;
+  double bandwidth, doppler, los_doppler, tx_scale;
+  const int32_t* delays;
+  const double* path_gains;
+  const double* corr_rx;
+  const double* corr_tx;
+  const double* noise_power;
+  uint64_t seed;
+  int64_t first_drop;
+  float2* spatial;  // [drop][nr][nt]
+  float2* coef;     // [drop][L][nb][5]
+  float2* abar;     // [drop][L][words]
+  float2* det_v;    // [drop][nt][nt]
+  float2* det_a;    // [drop][nt][nr]
+  float* det_l;     // [drop][nt]
+  float2* z;        // [drop][nt][words][n]
+  float2* grid;     // [drop][words][nr][active]
+  float2* csi;      // [drop][words][active]
+  int32_t* errors;  // [drop], accumulated
+  float2* equalized;  // [drop][nt][words][active] or null
+  int k0_uni, k0_sos, k0_e, k0_f, k0_s64, k0_det, k0_lds;
+  int k1_x, k1_tw, k1_coef, k1_bits, k1_cons, k1_lds;
+  int k2_y, k2_tw, k2_lds;
+};
+
+__device__ __forceinline__ uint32_t symbol_bits_at(const uint32_t* bw, int bit, int bps) {
+  uint64_t v = bw[bit >> 5];
+  if ((bit & 31) + bps > 32) v |= static_cast<uint64_t>(bw[(bit >> 5) + 1]) << 32;
+  return static_cast<uint32_t>(v >> (bit & 31)) & ((1u << bps) - 1u);
+}
+
+// ---------------------------------------------------------------- K0: per-drop prologue
+__global__ void __launch_bounds__(256) massive_prologue_kernel(MassiveArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  double* uni = reinterpret_cast<double*>(lds + a.k0_uni);
+  double* sos = reinterpret_cast<double*>(lds + a.k0_sos);
+  double* e = reinterpret_cast<double*>(lds + a.k0_e);    // exp(2 pi j U) [nr][nt]
+  double* f = reinterpret_cast<double*>(lds + a.k0_f);    // E R_tx
+  double* s64 = reinterpret_cast<double*>(lds + a.k0_s64);
+  double* det = reinterpret_cast<double*>(lds + a.k0_det);
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int64_t ld = blockIdx.x;
+  const uint32_t drop = static_cast<uint32_t>(a.first_drop + ld);
+  const int nr = a.nr, nt = a.nt, L = a.paths;
+  for (int c = tid; 2 * c < a.nnorm; c += nth) {
+    const double2 zz = normal_pair_f64(stream_call(a.seed, drop, HPX_STREAM_REALIZATION, static_cast<uint32_t>(c)));
+    uni[2 * c] = normal_cdf(zz.x);
+    if (2 * c + 1 < a.nnorm) uni[2 * c + 1] = normal_cdf(zz.y);
+  }
+  __syncthreads();
+  for (int i = tid; i < nr * nt; i += nth) {
+    const int p = i / nt, q = i - p * nt;
+    double sn, cs;
+    sincospi(2.0 * uni[p * a.blk_cols + q], &sn, &cs);
+    e[2 * i] = cs;
+    e[2 * i + 1] = sn;
+  }
+  __syncthreads();
+  // F = E R_tx, S = R_rx F (fading.py:476-489)
+  for (int i = tid; i < nr * nt; i += nth) {
+    const int p = i / nt, t = i - p * nt;
+    double fr = e[2 * i], fi = e[2 * i + 1];
+    if (a.corr_tx) {
+      fr = fi = 0.0;
+      for (int q = 0; q < nt; ++q) {
+        const double er = e[2 * (p * nt + q)], ei = e[2 * (p * nt + q) + 1];
+        const double br = a.corr_tx[2 * (q * nt + t)], bi = a.corr_tx[2 * (q * nt + t) + 1];
+        fr += er * br - ei * bi;
+        fi += er * bi + ei * br;
+      }
+    }
+    f[2 * i] = fr;
+    f[2 * i + 1] = fi;
+  }
+  __syncthreads();
+  for (int i = tid; i < nr * nt; i += nth) {
+    const int r = i / nt, t = i - r * nt;
+    double sr = f[2 * i], si = f[2 * i + 1];
+    if (a.corr_rx) {
+      sr = si = 0.0;
+      for (int p = 0; p < nr; ++p) {
+        const double ar = a.corr_rx[2 * (r * nr + p)], ai = a.corr_rx[2 * (r * nr + p) + 1];
+        const double br = f[2 * (p * nt + t)], bi = f[2 * (p * nt + t) + 1];
+        sr += ar * br - ai * bi;
+        si += ar * bi + ai * br;
+      }
+    }
+    s64[2 * i] = sr;
+    s64[2 * i + 1] = si;
+    a.spatial[ld * nr * nt + i] = make_float2(static_cast<float>(sr), static_cast<float>(si));
+  }
+  // SoS table (fading.py:326-342, angles/phases :492-499): NLOS terms, then LOS
+  {
+    const int ns = a.sinusoids;
+    const int o0 = (nr > 10 ? nr : 10) * a.blk_cols;
+    const int o_los_ang = o0, o_nlos_ang = o0 + L, o_los_ph = o0 + L + L * ns, o_nlos_ph = o0 + 2 * L + L * ns;
+    const double inv_fs = 1.0 / a.bandwidth;
+    for (int i = tid; i < L * a.terms; i += nth) {
+      const int p = i / a.terms, k = i - p * a.terms;
+      double* row = sos + 3 * i;
+      if (k < ns) {
+        const double ang = -M_PI + 2.0 * M_PI * uni[o_nlos_ang + p * ns + k];
+        row[0] = a.doppler * cos((2.0 * M_PI * (k + 1) + ang) / ns) * inv_fs;
+        row[1] = -M_PI + 2.0 * M_PI * uni[o_nlos_ph + p * ns + k];
+        row[2] = a.path_gains[L + p];
+      } else {
+        const double ang = 2.0 * M_PI * uni[o_los_ang + p];
+        row[0] = a.los_doppler * cos(ang) * inv_fs;
+        row[1] = -M_PI + 2.0 * M_PI * uni[o_los_ph + p];
+        row[2] = a.path_gains[p];
+      }
+    }
+  }
+  __syncthreads();
+  const TaylorBlocks tb{a.log2b};
+  float2* coef = a.coef + ld * L * a.nb * kTaylorCoeffs;
+  for (int i = tid; i < L * a.nb; i += nth) {
+    const int p = i / a.nb, b = i - p * a.nb;
+    taylor_coeffs(sos + 3 * p * a.terms, a.terms, tb.centre(b), tb.half_width(), coef + i * kTaylorCoeffs);
+  }
+  for (int i = tid; i < L * a.words; i += nth) {
+    const int p = i / a.words, w = i - p * a.words;
+    const double2 m = tap_mean(sos + 3 * p * a.terms, a.terms, static_cast<double>(w) * (a.n + a.cp) + a.cp, a.n);
+    a.abar[ld * L * a.words + i] = make_float2(static_cast<float>(m.x), static_cast<float>(m.y));
+  }
+  if (tid == 0) {
+    double* lam = det;
+    double* v = lam + nt;
+    double* am = v + 2 * nt * nt;
+    spatial_detector(nr, nt, s64, lam, v, am, am + 2 * nt * nr);
+    for (int i = 0; i < nt * nt; ++i)
+      a.det_v[ld * nt * nt + i] = make_float2(static_cast<float>(v[2 * i]), static_cast<float>(v[2 * i + 1]));
+    for (int i = 0; i < nt * nr; ++i)
+      a.det_a[ld * nt * nr + i] = make_float2(static_cast<float>(am[2 * i]), static_cast<float>(am[2 * i + 1]));
+    double lmax = 0.0;
+    for (int i = 0; i < nt; ++i) lmax = fmax(lmax, lam[i]);
+    const double rt = static_cast<double>(nr > nt ? nr : nt) * 2.220446049250313e-16;
+    for (int i = 0; i < nt; ++i) a.det_l[ld * nt + i] = lam[i] > rt * rt * lmax ? static_cast<float>(lam[i]) : 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------- K1: transmitter + delay line
+__global__ void __launch_bounds__(kTxThreads) massive_tx_kernel(MassiveArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  float2* x = reinterpret_cast<float2*>(lds + a.k1_x);
+  float2* tw = reinterpret_cast<float2*>(lds + a.k1_tw);
+  float2* coef = reinterpret_cast<float2*>(lds + a.k1_coef);
+  uint32_t* bits = reinterpret_cast<uint32_t*>(lds + a.k1_bits);
+  float2* cons = reinterpret_cast<float2*>(lds + a.k1_cons);
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int64_t ld = blockIdx.x / a.nt;
+  const int t = static_cast<int>(blockIdx.x - ld * a.nt);
+  const uint32_t drop = static_cast<uint32_t>(a.first_drop + ld);
+  const int n = a.n, L = a.paths, x0 = a.x0;
+  const int ncoef = L * a.nb * kTaylorCoeffs;
+  for (int i = tid; i < ncoef; i += nth) coef[i] = a.coef[ld * ncoef + i];
+  for (int v = tid; v < a.order; v += nth) {
+    const double2 p = constellation_point(a.order, 1, bits_to_label(static_cast<uint32_t>(v), a.bps));
+    cons[v] = make_float2(static_cast<float>(p.x * a.tx_scale), static_cast<float>(p.y * a.tx_scale));
+  }
+  for (int k = tid; k < n; k += nth) {
+    double sn, cs;
+    sincospi(-2.0 * static_cast<double>(k) / static_cast<double>(n), &sn, &cs);
+    tw[k] = make_float2(static_cast<float>(cs), static_cast<float>(sn));
+  }
+  for (int i = tid; i < a.dx; i += nth) x[i] = make_float2(0.f, 0.f);  // silence before the frame
+  __syncthreads();
+  const TaylorBlocks tb{a.log2b};
+  const int period = n + a.cp;
+  float2* zt = a.z + (ld * a.nt + t) * static_cast<int64_t>(a.words) * n;
+  for (int w = 0; w < a.words; ++w) {
+    // the word's data bits (stream HPX_STREAM_BITS + t, LSB-first words)
+    const int64_t bit0 = static_cast<int64_t>(w) * a.active * a.bps;
+    const int bo = static_cast<int>(bit0 & 31);
+    const int fw = static_cast<int>(bit0 >> 5);
+    const int lw = static_cast<int>((bit0 + static_cast<int64_t>(a.active) * a.bps - 1) >> 5);
+    for (int c = (fw >> 2) + tid; c <= (lw >> 2); c += nth) {
+      const uint4 r = stream_call(a.seed, drop, HPX_STREAM_BITS + t, static_cast<uint32_t>(c));
+      const uint32_t comp[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int wi = 4 * c + k - fw;
+        if (wi >= 0 && wi <= lw - fw) bits[wi] = comp[k];
+      }
+    }
+    __syncthreads();
+    for (int q = tid; q < n; q += nth) {
+      const int el = q - a.active_lo;
+      x[x0 + subcarrier_bin(q, n, a.dc)] =
+          (el >= 0 && el < a.active) ? cons[symbol_bits_at(bits, bo + el * a.bps, a.bps)] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    lds_fft<+1>(x + x0, 1, a.xs, a.log2n, tw);
+    for (int i = tid; i < a.cpl; i += nth) x[x0 - a.cpl + i] = x[x0 + n - a.cpl + i];
+    __syncthreads();
+    // tapped delay line over the data window (the receiver keeps nothing else)
+    const int64_t win0 = static_cast<int64_t>(w) * period + a.cp;
+    for (int nl = tid; nl < n; nl += nth) {
+      const int64_t nabs = win0 + nl;
+      float2 acc = make_float2(0.f, 0.f);
+      for (int p = 0; p < L; ++p) {
+        const int d = a.delays[p];
+        const int64_t m = nabs - d;
+        if (d < 0 || d > a.maxd || m < 0) continue;
+        const float2 h = taylor_eval(coef + (p * a.nb + static_cast<int>(tb.block_of(m))) * kTaylorCoeffs,
+                                     tb.offset(m));
+        acc = cfma(h, x[x0 + nl - d], acc);
+      }
+      zt[static_cast<int64_t>(w) * n + nl] = acc;
+    }
+    // the last dx data samples feed the next word's delayed paths
+    float2 tail[kTail];
+#pragma unroll
+    for (int k = 0; k < kTail; ++k) {
+      const int i = tid + k * nth;
+      if (i < a.dx) tail[k] = x[x0 + n - a.dx + i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kTail; ++k) {
+      const int i = tid + k * nth;
+      if (i < a.dx) x[i] = tail[k];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- K2: receive antennas
+__global__ void __launch_bounds__(kRxThreads) massive_rx_kernel(MassiveArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  float2* y = reinterpret_cast<float2*>(lds + a.k2_y);  // [kRxRows + 1][n]
+  float2* tw = reinterpret_cast<float2*>(lds + a.k2_tw);
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int groups = (a.nr + kRxGroup - 1) / kRxGroup;
+  const int64_t blk = blockIdx.x;
+  const int g = static_cast<int>(blk % groups);
+  const int w = static_cast<int>((blk / groups) % a.words);
+  const int64_t ld = blk / (static_cast<int64_t>(groups) * a.words);
+  const uint32_t drop = static_cast<uint32_t>(a.first_drop + ld);
+  const int n = a.n, nt = a.nt;
+  for (int k = tid; k < n; k += nth) {
+    double sn, cs;
+    sincospi(-2.0 * static_cast<double>(k) / static_cast<double>(n), &sn, &cs);
+    tw[k] = make_float2(static_cast<float>(cs), static_cast<float>(sn));
+  }
+  const float sigma = static_cast<float>(sqrt(0.5 * a.noise_power[ld]));
+  const float rx_scale = static_cast<float>(1.0 / sqrt(static_cast<double>(n)));
+  const int64_t win0 = static_cast<int64_t>(w) * (n + a.cp) + a.cp;
+  const float2* zd = a.z + ld * nt * static_cast<int64_t>(a.words) * n + static_cast<int64_t>(w) * n;
+  const float2* sp = a.spatial + ld * a.nr * nt;
+  const int r_end = std::min(a.nr, (g + 1) * kRxGroup);
+  const bool with_csi = g == 0;
+  for (int r0 = g * kRxGroup; r0 < r_end; r0 += kRxRows) {
+    const int rows = std::min(kRxRows, r_end - r0);
+    __syncthreads();
+    for (int i = tid; i < rows * n; i += nth) {
+      const int rr = i / n, nl = i - rr * n;
+      const int r = r0 + rr;
+      float2 acc = make_float2(0.f, 0.f);
+      for (int t = 0; t < nt; ++t) acc = cfma(sp[r * nt + t], zd[static_cast<int64_t>(t) * a.words * n + nl], acc);
+      // noise sample nabs: words (0, 1) or (2, 3) of Philox call nabs / 2 (stream HPX_STREAM_NOISE + r)
+      const int64_t nabs = win0 + nl;
+      const uint4 wn = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>(nabs >> 1));
+      const float2 nz = (nabs & 1) ? complex_normal_f32(wn.z, wn.w) : complex_normal_f32(wn.x, wn.y);
+      y[rr * n + nl] = make_float2(fmaf(sigma, nz.x, acc.x), fmaf(sigma, nz.y, acc.y));
+    }
+    const bool csi_now = with_csi && r0 == g * kRxGroup;
+    if (csi_now) {
+      for (int i = tid; i < n; i += nth) y[rows * n + i] = make_float2(0.f, 0.f);
+      __syncthreads();
+      if (tid == 0) {
+        const float2* ab = a.abar + ld * a.paths * a.words;
+        for (int p = 0; p < a.paths; ++p) {
+          const int d = a.delays[p];
+          if (d >= 0 && d < a.keep_below) y[rows * n + d] = cadd(y[rows * n + d], ab[p * a.words + w]);
+        }
+      }
+    }
+    __syncthreads();
+    lds_fft<-1>(y, rows + (csi_now ? 1 : 0), n, a.log2n, tw);
+    for (int i = tid; i < rows * a.active; i += nth) {
+      const int rr = i / a.active, k = i - rr * a.active;
+      const int bin = subcarrier_bin(a.active_lo + k, n, a.dc);
+      a.grid[((ld * a.words + w) * a.nr + r0 + rr) * static_cast<int64_t>(a.active) + k] =
+          cscale(y[rr * n + bin], rx_scale);
+    }
+    if (csi_now) {
+      for (int k = tid; k < a.active; k += nth)
+        a.csi[(ld * a.words + w) * static_cast<int64_t>(a.active) + k] =
+            y[rows * n + subcarrier_bin(a.active_lo + k, n, a.dc)];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- K3: detection (MFMA)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// u[e][i] = sum_r A[i][r] y[e][r] as a real GEMM: rows = elements, K = 2 nr (re parts, then im
+// parts of y), columns 0..nt-1 = Re u, nt..2nt-1 = Im u (zero-padded to 32).  Both operands are
+// split into bfloat16 hi + lo; hi*hi + hi*lo + lo*hi on 32x32x8 MFMAs keeps ~16 mantissa bits.
+constexpr int kDetK = 2 * kMaxRx;        // 128
+constexpr int kDetKs = kDetK + 4;        // padded LDS row
+
+__global__ void __launch_bounds__(kDetThreads) massive_detect_kernel(MassiveArgs a) {
+  __shared__ __align__(16) uint16_t yh[kDetElements * kDetKs];
+  __shared__ __align__(16) uint16_t yl[kDetElements * kDetKs];
+  __shared__ __align__(16) uint16_t bh[32 * kDetKs];
+  __shared__ __align__(16) uint16_t bl[32 * kDetKs];
+  __shared__ float us[kDetElements][2 * kMaxUsers + 1];
+  __shared__ int red[kDetThreads / 64];
+  const int tid = threadIdx.x;
+  const int tiles = (a.active + kDetElements - 1) / kDetElements;
+  const int64_t blk = blockIdx.x;
+  const int tile = static_cast<int>(blk % tiles);
+  const int w = static_cast<int>((blk / tiles) % a.words);
+  const int64_t ld = blk / (static_cast<int64_t>(tiles) * a.words);
+  const uint32_t drop = static_cast<uint32_t>(a.first_drop + ld);
+  const int nr = a.nr, nt = a.nt;
+  const int k0 = tile * kDetElements;
+  // B[col][k]: col < nt -> Re u_col: k < nr: Re A, else -Im A;  nt <= col < 2 nt -> Im u: Im A, Re A
+  const float2* am = a.det_a + ld * nt * nr;
+  for (int i = tid; i < 32 * kDetK; i += kDetThreads) {
+    const int col = i / kDetK, k = i - col * kDetK;
+    float v = 0.0f;
+    const int r = k < nr ? k : k - kMaxRx;
+    if (r >= 0 && r < nr && (k < nr || k >= kMaxRx)) {
+      if (col < nt) {
+        const float2 x = am[col * nr + r];
+        v = k < kMaxRx ? x.x : -x.y;
+      } else if (col < 2 * nt) {
+        const float2 x = am[(col - nt) * nr + r];
+        v = k < kMaxRx ? x.y : x.x;
+      }
+    }
+    const uint16_t h = f2bf(v);
+    bh[col * kDetKs + k] = h;
+    bl[col * kDetKs + k] = f2bf(v - bf2f(h));
+  }
+  // y tile: element e (row), k = r (re) or kMaxRx + r (im)
+  const float2* gy = a.grid + (ld * a.words + w) * static_cast<int64_t>(nr) * a.active;
+  for (int i = tid; i < kDetElements * kMaxRx; i += kDetThreads) {
+    const int r = i / kDetElements, e = i - r * kDetElements;  // consecutive threads: consecutive elements
+    const int k = k0 + e;
+    float2 v = make_float2(0.f, 0.f);
+    if (r < nr && k < a.active) v = gy[static_cast<int64_t>(r) * a.active + k];
+    uint16_t h = f2bf(v.x);
+    yh[e * kDetKs + r] = h;
+    yl[e * kDetKs + r] = f2bf(v.x - bf2f(h));
+    h = f2bf(v.y);
+    yh[e * kDetKs + kMaxRx + r] = h;
+    yl[e * kDetKs + kMaxRx + r] = f2bf(v.y - bf2f(h));
+  }
+  __syncthreads();
+  {
+    const int wave = tid >> 6, lane = tid & 63;
+    const int m = lane & 31, kq = lane >> 5;
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    const uint16_t* ah = yh + (32 * wave + m) * kDetKs + 4 * kq;
+    const uint16_t* al = yl + (32 * wave + m) * kDetKs + 4 * kq;
+    const uint16_t* bhh = bh + m * kDetKs + 4 * kq;
+    const uint16_t* bll = bl + m * kDetKs + 4 * kq;
+#pragma unroll 4
+    for (int k = 0; k < kDetK; k += 8) {
+      const bf16x4 a_h = *reinterpret_cast<const bf16x4*>(ah + k);
+      const bf16x4 a_l = *reinterpret_cast<const bf16x4*>(al + k);
+      const bf16x4 b_h = *reinterpret_cast<const bf16x4*>(bhh + k);
+      const bf16x4 b_l = *reinterpret_cast<const bf16x4*>(bll + k);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a_l, b_h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a_h, b_l, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a_h, b_h, acc, 0, 0, 0);
+    }
+    // D[row][col]: row = element 32 wave + (r / 4) 8 + 4 kq + r % 4, col = m
+    if (m < 2 * kMaxUsers) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) us[32 * wave + (r >> 2) * 8 + kq * 4 + (r & 3)][m] = acc[r];
+    }
+  }
+  __syncthreads();
+  // per element: x = V diag(conj(c) / (|c|^2 lambda + s2)) u, decisions, bit errors
+  int err = 0;
+  const float s2 = static_cast<float>(a.noise_power[ld]);
+  const float2* dv = a.det_v + ld * nt * nt;
+  const float* dl = a.det_l + ld * nt;
+  for (int e = tid; e < kDetElements; e += kDetThreads) {
+    const int k = k0 + e;
+    if (k >= a.active) continue;
+    const float2 c = a.csi[(ld * a.words + w) * static_cast<int64_t>(a.active) + k];
+    const float c2 = cabs2(c);
+    float2 u[kMaxUsers];
+    for (int i = 0; i < nt; ++i) {
+      const float den = fmaf(c2, dl[i], s2);
+      const float gi = (dl[i] > 0.0f || s2 > 0.0f) && den > 0.0f ? 1.0f / den : 0.0f;
+      u[i] = cscale(cmulc(c, make_float2(us[e][i], us[e][nt + i])), gi);
+    }
+    const int64_t sym = static_cast<int64_t>(w) * a.active + k;
+    for (int t = 0; t < nt; ++t) {
+      float2 xh = make_float2(0.f, 0.f);
+      for (int i = 0; i < nt; ++i) xh = cfma(dv[t * nt + i], u[i], xh);
+      const uint32_t rx = demap_symbol(a.mod, static_cast<double>(xh.x), static_cast<double>(xh.y));
+      // transmitted bits of symbol sym of user t
+      const int64_t pos = sym * a.bps;
+      const int64_t wi = pos >> 5;
+      const uint4 q = stream_call(a.seed, drop, HPX_STREAM_BITS + t, static_cast<uint32_t>(wi >> 2));
+      const uint32_t comp[4] = {q.x, q.y, q.z, q.w};
+      uint64_t v = comp[wi & 3];
+      const int sh = static_cast<int>(pos & 31);
+      if (sh + a.bps > 32) {
+        const int64_t wn = wi + 1;
+        const uint32_t nx = (wn & 3) ? comp[wn & 3]
+                                     : stream_call(a.seed, drop, HPX_STREAM_BITS + t, static_cast<uint32_t>(wn >> 2)).x;
+        v |= static_cast<uint64_t>(nx) << 32;
+      }
+      const uint32_t tx = static_cast<uint32_t>(v >> sh) & ((1u << a.bps) - 1u);
+      err += __popc(rx ^ tx);
+      if (a.equalized) a.equalized[((ld * nt + t) * a.words + w) * static_cast<int64_t>(a.active) + k] = xh;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) err += __shfl_down(err, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = err;
+  __syncthreads();
+  if (tid == 0) {
+    int s = 0;
+    for (int i = 0; i < kDetThreads / 64; ++i) s += red[i];
+    if (s) atomicAdd(a.errors + ld, s);
+  }
+}
+
+int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
+
+int make_massive(const hpx_massive_link* l, MassiveArgs* out) {
+  HPX_REQUIRE(l, "hpx_massive_run: null link");
+  MassiveArgs& a = *out;
+  a = MassiveArgs{};
+  HPX_REQUIRE(l->num_rx >= 1 && l->num_rx <= kMaxRx && l->num_users >= 1 && l->num_users <= kMaxUsers &&
+                  l->num_users <= l->num_rx,
+              "hpx_massive_run: needs 1 <= users <= %d, users <= antennas <= %d", kMaxUsers, kMaxRx);
+  a.log2n = ilog2_exact(l->num_subcarriers);
+  if (a.log2n < 6 || l->num_subcarriers > 4096) {
+    set_error("hpx_massive_run: num_subcarriers must be a power of two in [64, 4096]");
+    return HPX_ERR_UNSUPPORTED;
+  }
+  HPX_REQUIRE(l->active >= 1 && l->active <= l->num_subcarriers && l->words >= 1 && l->prefix >= 0 &&
+                  l->prefix <= l->num_subcarriers && l->num_paths >= 1 && l->num_sinusoids >= 1 && l->max_delay >= 0 &&
+                  l->bandwidth > 0.0 && l->device_power >= 0.0,
+              "hpx_massive_run: invalid link");
+  if (l->modulation_order == 2 || !modulation(l->modulation_order, 1, &a.mod)) {
+    set_error("hpx_massive_run: modulation order %d not supported (4, 16, 64, 256)", l->modulation_order);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  a.nr = l->num_rx;
+  a.nt = l->num_users;
+  a.n = l->num_subcarriers;
+  a.cp = l->prefix;
+  a.words = l->words;
+  a.active = l->active;
+  a.active_lo = l->num_subcarriers / 2 - l->active / 2;
+  a.bps = a.mod.bps;
+  a.order = l->modulation_order;
+  a.dc = l->dc_suppression ? 1 : 0;
+  a.paths = l->num_paths;
+  a.sinusoids = l->num_sinusoids;
+  a.terms = l->num_sinusoids + 1;
+  a.blk_cols = std::max(10, l->num_users);
+  a.nnorm = std::max(10, l->num_rx) * a.blk_cols + 2 * l->num_paths + 2 * l->num_paths * l->num_sinusoids;
+  a.frame_len = static_cast<int64_t>(l->words) * (l->num_subcarriers + l->prefix);
+  a.bandwidth = l->bandwidth;
+  a.doppler = l->doppler;
+  a.los_doppler = l->los_doppler;
+  a.tx_scale = sqrt(l->device_power) / sqrt(static_cast<double>(l->num_subcarriers));
+  const double max_rate = std::max(fabs(l->doppler), fabs(l->los_doppler)) / l->bandwidth;
+  a.log2b = taylor_log2_block(max_rate, a.terms, 11);
+  if (a.log2b < 6) {
+    set_error("hpx_massive_run: Doppler rate %.3g rad/sample too high for the block Taylor taps", max_rate);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  a.nb = static_cast<int>((a.frame_len + (int64_t(1) << a.log2b) - 1) >> a.log2b);
+  a.maxd = l->max_delay;
+  a.dx = std::max(0, l->max_delay - a.cp);
+  HPX_REQUIRE(a.dx <= kTail * kTxThreads && a.dx <= a.n, "hpx_massive_run: delay spread too long");
+  a.cpl = std::min(a.cp, a.maxd);
+  a.x0 = a.dx + a.cpl;
+  a.xs = a.x0 + a.n;
+  a.keep_below = static_cast<int>(std::min<int64_t>(std::min<int64_t>(1 + l->max_delay, a.frame_len), a.n));
+  a.bits_words = (a.active * a.bps + 63) >> 5;
+  // K0 LDS
+  int64_t o = 0;
+  a.k0_uni = 0;
+  o += align16(static_cast<int64_t>(a.nnorm) * 8);
+  a.k0_sos = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.paths) * a.terms * 3 * 8);
+  a.k0_e = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.nr) * a.nt * 16);
+  a.k0_f = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.nr) * a.nt * 16);
+  a.k0_s64 = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.nr) * a.nt * 16);
+  a.k0_det = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.nt + 2 * a.nt * a.nt + 2 * a.nt * a.nr + kEigWork) * 8);
+  a.k0_lds = static_cast<int>(o);
+  // K1 LDS
+  o = 0;
+  a.k1_x = 0;
+  o += align16(static_cast<int64_t>(a.xs) * 8);
+  a.k1_tw = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.n) * 8);
+  a.k1_coef = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs * 8);
+  a.k1_bits = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.bits_words) * 4);
+  a.k1_cons = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.order) * 8);
+  a.k1_lds = static_cast<int>(o);
+  // K2 LDS
+  a.k2_y = 0;
+  a.k2_tw = static_cast<int>(align16(static_cast<int64_t>(kRxRows + 1) * a.n * 8));
+  a.k2_lds = static_cast<int>(a.k2_tw + align16(static_cast<int64_t>(a.n) * 8));
+  if (a.k0_lds > 160 * 1024 || a.k1_lds > 160 * 1024 || a.k2_lds > 160 * 1024) {
+    set_error("hpx_massive_run: the link needs more than 160 KB of LDS per workgroup");
+    return HPX_ERR_UNSUPPORTED;
+  }
+  if (a.n > kFftMaxPerThread * kRxThreads) {
+    set_error("hpx_massive_run: %d subcarriers exceed one workgroup's FFT capacity", a.n);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  return HPX_OK;
+}
+
+bool raise_lds(const void* fn, int bytes) {
+  if (bytes <= 64 * 1024) return true;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int hpx_massive_sizes(const hpx_massive_link* link_host, int64_t* sizes_host) {
+  HPX_REQUIRE(sizes_host, "hpx_massive_sizes: null pointer");
+  MassiveArgs a;
+  const int st = make_massive(link_host, &a);
+  if (st != HPX_OK) return st;
+  sizes_host[0] = static_cast<int64_t>(a.nr) * a.nt;                        // spatial
+  sizes_host[1] = static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs;     // coef
+  sizes_host[2] = static_cast<int64_t>(a.paths) * a.words;                  // abar
+  sizes_host[3] = static_cast<int64_t>(a.nt) * a.nt;                        // det_v
+  sizes_host[4] = static_cast<int64_t>(a.nt) * a.nr;                        // det_a
+  sizes_host[5] = a.nt;                                                     // det_l (float32)
+  sizes_host[6] = static_cast<int64_t>(a.nt) * a.words * a.n;               // z
+  sizes_host[7] = static_cast<int64_t>(a.words) * a.nr * a.active;          // grid
+  sizes_host[8] = static_cast<int64_t>(a.words) * a.active;                 // csi
+  return HPX_OK;
+}
+
+extern "C" int hpx_massive_run(const hpx_massive_link* link_host, const int32_t* delays, const double* path_gains,
+                               const hpx_c128* corr_rx, const hpx_c128* corr_tx, uint64_t seed, int64_t first_drop,
+                               int64_t num_drops, const double* noise_power, const hpx_massive_work* work_host,
+                               int32_t* errors, hpx_c64* equalized, void* stream) {
+  MassiveArgs a;
+  const int st = make_massive(link_host, &a);
+  if (st != HPX_OK) return st;
+  HPX_REQUIRE(num_drops >= 0 && num_drops <= 65535, "hpx_massive_run: 0..65535 drops per call");
+  HPX_REQUIRE(first_drop >= 0 && first_drop + num_drops <= (int64_t(1) << 32), "hpx_massive_run: drop index exceeds 32 bits");
+  if (num_drops == 0) return HPX_OK;
+  HPX_REQUIRE(delays && path_gains && noise_power && work_host && errors, "hpx_massive_run: null pointer");
+  const hpx_massive_work& wk = *work_host;
+  HPX_REQUIRE(wk.spatial && wk.coef && wk.abar && wk.det_v && wk.det_a && wk.det_l && wk.z && wk.grid && wk.csi,
+              "hpx_massive_run: null work buffer");
+  a.delays = delays;
+  a.path_gains = path_gains;
+  a.corr_rx = reinterpret_cast<const double*>(corr_rx);
+  a.corr_tx = reinterpret_cast<const double*>(corr_tx);
+  a.noise_power = noise_power;
+  a.seed = seed;
+  a.first_drop = first_drop;
+  a.spatial = reinterpret_cast<float2*>(wk.spatial);
+  a.coef = reinterpret_cast<float2*>(wk.coef);
+  a.abar = reinterpret_cast<float2*>(wk.abar);
+  a.det_v = reinterpret_cast<float2*>(wk.det_v);
+  a.det_a = reinterpret_cast<float2*>(wk.det_a);
+  a.det_l = wk.det_l;
+  a.z = reinterpret_cast<float2*>(wk.z);
+  a.grid = reinterpret_cast<float2*>(wk.grid);
+  a.csi = reinterpret_cast<float2*>(wk.csi);
+  a.errors = errors;
+  a.equalized = reinterpret_cast<float2*>(equalized);
+  const hipStream_t s = as_stream(stream);
+  if (!raise_lds(reinterpret_cast<const void*>(massive_prologue_kernel), a.k0_lds) ||
+      !raise_lds(reinterpret_cast<const void*>(massive_tx_kernel), a.k1_lds) ||
+      !raise_lds(reinterpret_cast<const void*>(massive_rx_kernel), a.k2_lds)) {
+    set_error("hpx_massive_run: LDS limit not available");
+    return HPX_ERR_UNSUPPORTED;
+  }
+  const unsigned nd = static_cast<unsigned>(num_drops);
+  hipLaunchKernelGGL(massive_prologue_kernel, dim3(nd), dim3(256), a.k0_lds, s, a);
+  hipLaunchKernelGGL(massive_tx_kernel, dim3(nd * a.nt), dim3(kTxThreads), a.k1_lds, s, a);
+  const unsigned groups = static_cast<unsigned>((a.nr + kRxGroup - 1) / kRxGroup);
+  hipLaunchKernelGGL(massive_rx_kernel, dim3(nd * a.words * groups), dim3(kRxThreads), a.k2_lds, s, a);
+  const unsigned tiles = static_cast<unsigned>((a.active + kDetElements - 1) / kDetElements);
+  hipLaunchKernelGGL(massive_detect_kernel, dim3(nd * a.words * tiles), dim3(kDetThreads), 0, s, a);
+  return check_launch("hpx_massive_run");
+}

@@ -5,7 +5,9 @@ transmit) stream pair maps the LS values at that stream's pilots (two pilot
 words, real and imaginary parts, plus a stream flag: 5 of 8 input channels) to
 the channel of every resource element: 64 positions x (words x N / 64
 subcarriers x re/im) outputs.  Four layers, kernel 5, 64 hidden channels, GELU
-(tanh form): about 115k parameters.  Weights and activations are bfloat16,
+(tanh form), plus a linear kernel-5 skip convolution from the input to the
+output (the local linear interpolator the nonlinear path refines): about 125k
+parameters.  Weights and activations are bfloat16,
 products accumulate in float32 on the MFMA units (``hpx_conv1d_forward``);
 training runs on the device with hand-written kernels: backward-data as the
 forward convolution with transposed, flipped weights, float32 weight
@@ -43,8 +45,15 @@ def layer_shapes(cfg: LinkConfig) -> list:
     return [(CIN, HIDDEN, True), (HIDDEN, HIDDEN, True), (HIDDEN, HIDDEN, True), (HIDDEN, cout, False)]
 
 
+def skip_shape(cfg: LinkConfig) -> tuple:
+    """(cin, cout) of the linear skip convolution."""
+
+    return CIN, layer_shapes(cfg)[-1][1]
+
+
 def init_params(cfg: LinkConfig, seed: int = 1) -> list:
-    """Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights [cout][5][cin] and biases, float32."""
+    """Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights [cout][5][cin] and biases of the four layers,
+    then the skip convolution, zero-initialised, float32."""
 
     rng = np.random.default_rng(seed)
     out = []
@@ -53,6 +62,8 @@ def init_params(cfg: LinkConfig, seed: int = 1) -> list:
         w = rng.uniform(-bound, bound, (cout, TAPS, cin)).astype(np.float32)
         b = rng.uniform(-bound, bound, cout).astype(np.float32)
         out.append((w, b))
+    cin, cout = skip_shape(cfg)
+    out.append((np.zeros((cout, TAPS, cin), np.float32), np.zeros(cout, np.float32)))
     return out
 
 
@@ -122,10 +133,36 @@ class ConvNetEstimator:
             cache.append((a, pre))
             a = nxt
             out = pre
+        # linear skip path from the input, accumulated onto the last layer's output
+        cin, cout = skip_shape(self.cfg)
+        _lib.check(lib.hpx_conv1d_forward(_lib.ptr(x), s, cin, _lib.ptr(self.w16[-1]), _lib.ptr(self.b[-1]), cout, 2,
+                                          _lib.ptr(out), None, st))
         return (out, cache) if keep else out
 
+    def estimate_grid(self, grid: torch.Tensor, pilot_index: torch.Tensor, pilot_values: torch.Tensor,
+                      out: torch.Tensor | None = None) -> torch.Tensor:
+        """Channel estimates [num_tx][drops][num_rx][words * N] of received grids [drops][num_rx][words][N]
+        in one launch (``hpx_learned_estimate``: pilot LS, the network and the skip path with the
+        activations in LDS)."""
+
+        import ctypes
+
+        c = self.cfg
+        drops = grid.shape[0]
+        if out is None:
+            out = torch.empty((c.num_tx, drops, c.num_rx, c.words * c.num_subcarriers), dtype=torch.complex64,
+                              device=self.device)
+        ws = (ctypes.c_void_p * 5)(*[w.data_ptr() for w in self.w16])
+        bs = (ctypes.c_void_p * 5)(*[b.data_ptr() for b in self.b])
+        _lib.check(_lib.load().hpx_learned_estimate(
+            _lib.ptr(grid), drops, c.num_rx, c.num_tx, c.words, c.num_subcarriers, _lib.ptr(pilot_index),
+            _lib.ptr(pilot_values), pilot_index.shape[1], ctypes.addressof(ws), ctypes.addressof(bs),
+            self.shapes[-1][1], _lib.ptr(out), _lib.stream_handle()))
+        return out
+
     def estimate(self, ls: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
-        """Channel estimates [num_tx][drops][num_rx][words * N] from LS values."""
+        """Channel estimates [num_tx][drops][num_rx][words * N] from LS values (layer by layer, the
+        training path's arithmetic)."""
 
         c = self.cfg
         num_tx, drops, num_rx, _ = ls.shape
@@ -148,7 +185,7 @@ class ConvNetEstimator:
     @classmethod
     def load(cls, cfg: LinkConfig, path: str | Path = WEIGHTS_PATH, device: str | torch.device = "cuda"):
         with np.load(path, allow_pickle=False) as f:
-            params = [(f[f"w{i}"], f[f"b{i}"]) for i in range(4)]
+            params = [(f[f"w{i}"], f[f"b{i}"]) for i in range(5)]
         return cls(cfg, params, device)
 
 
@@ -189,7 +226,9 @@ class Trainer:
         self.csi = torch.empty((drops_per_step, c.words, c.num_subcarriers), dtype=torch.complex64, device=dev)
         self.spatial = torch.empty((drops_per_step, c.num_rx, c.num_tx), dtype=torch.complex64, device=dev)
         self.errors = torch.empty(drops_per_step, dtype=torch.int32, device=dev)
-        self.acc = torch.zeros(2, dtype=torch.float64, device=dev)
+        self.num_groups = len(self.cfg.snr_db)
+        self.work = torch.zeros(3 * self.num_groups, dtype=torch.float64, device=dev)
+        self.loss = torch.zeros(1, dtype=torch.float64, device=dev)
 
     def simulate(self, first_drop: int):
         """Generate drops first_drop.. (SNR point i % len(snr_db), Doppler per drop): LS values and the truth."""
@@ -208,7 +247,8 @@ class Trainer:
         return ls
 
     def step(self, first_drop: int, lr: float | None = None) -> torch.Tensor:
-        """One Adam step on drops first_drop .. first_drop + drops_per_step; returns the batch NMSE (device)."""
+        """One Adam step on drops first_drop .. first_drop + drops_per_step; returns the loss, the mean
+        over SNR points of the NMSE in dB (device scalar)."""
 
         net, c = self.net, self.cfg
         lib, st = _lib.load(), _lib.stream_handle()
@@ -216,10 +256,12 @@ class Trainer:
         x = net.net_input(ls)
         out, cache = net.forward(x, keep=True)
         grad = torch.empty_like(out)
-        self.acc.zero_()
+        group = torch.as_tensor(((first_drop + np.arange(self.drops)) % self.num_groups).astype(np.int32),
+                                device=net.device)
+        self.work.zero_()
         _lib.check(lib.hpx_nmse_grad(_lib.ptr(out), _lib.ptr(self.csi), _lib.ptr(self.spatial), self.drops, c.num_rx,
-                                     c.num_tx, c.words, c.num_subcarriers, _lib.ptr(self.acc[0:1]), _lib.ptr(grad),
-                                     _lib.ptr(self.acc[1:2]), st))
+                                     c.num_tx, c.words, c.num_subcarriers, _lib.ptr(group), self.num_groups,
+                                     _lib.ptr(self.work), _lib.ptr(grad), _lib.ptr(self.loss), st))
         s = x.shape[0]
         for i in range(len(net.shapes) - 1, -1, -1):
             cin, cout, act = net.shapes[i]
@@ -231,6 +273,10 @@ class Trainer:
                 _lib.check(lib.hpx_to_bf16(_lib.ptr(grad), grad.numel(), _lib.ptr(dz), st))
             _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(a_in), s, cin, cout, _lib.ptr(self.workspace),
                                             _lib.ptr(self.gw[i]), _lib.ptr(self.gb[i]), st))
+            if i == len(net.shapes) - 1:  # the skip convolution sees the same output gradient
+                scin, scout = skip_shape(c)
+                _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(x), s, scin, scout, _lib.ptr(self.workspace),
+                                                _lib.ptr(self.gw[-1]), _lib.ptr(self.gb[-1]), st))
             if i > 0:
                 grad = torch.empty((s, POSITIONS, cin), dtype=torch.float32, device=net.device)
                 _lib.check(lib.hpx_conv1d_forward(_lib.ptr(dz), s, cout, _lib.ptr(net.wt16[i]), None, cin, 0,
@@ -241,7 +287,7 @@ class Trainer:
             _lib.check(lib.hpx_adam_step(_lib.ptr(p), _lib.ptr(g), _lib.ptr(m1), _lib.ptr(m2), p.numel(),
                                          float(self.lr if lr is None else lr), b1, b2, self.eps, self.step_count, st))
         net.refresh()
-        return self.acc[1]
+        return self.loss[0]
 
 
 def train(cfg: LinkConfig, num_drops: int, drops_per_step: int = 1024, lr: float = 2e-3, log_every: int = 50,
@@ -258,7 +304,7 @@ def train(cfg: LinkConfig, num_drops: int, drops_per_step: int = 1024, lr: float
         rate = lr * 0.5 * (1 + math.cos(math.pi * k / steps))
         loss = tr.step(first_drop + k * drops_per_step, rate)
         if verbose and (k % log_every == 0 or k == steps - 1):
-            print(f"step {k + 1}/{steps} nmse {10 * math.log10(max(float(loss), 1e-30)):.2f} dB "
+            print(f"step {k + 1}/{steps} mean nmse {float(loss):.2f} dB "
                   f"lr {rate:.2e} {time.perf_counter() - t0:.1f} s", flush=True)
         if save_to is not None and (k + 1) % save_every == 0:
             net.save(save_to)

@@ -1,0 +1,125 @@
+"""Massive-MIMO uplink drops on the device: 64 receive antennas, 8 single-antenna users,
+4096-point OFDM with 3276 active subcarriers (273 resource blocks of 12, centred on DC),
+CP 288, 14 words, 256-QAM, the correlated 23-tap TDL (rho = 0.7 at the receive array),
+LMMSE detection on ideal CSI.
+
+Beyond the reference's 10-antenna arrays (``hermespy/channel/fading/fading.py:476-489``);
+``hpx_massive_run`` documents how the spatial draw is extended.  A drop does not fit one
+CU, so it runs as four kernels over per-drop device buffers held here (prologue,
+transmit + delay line, receive FFTs, MFMA detection); see ``hermespy_amd/csrc/massive.hip``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .configs import SPEED_OF_LIGHT, LinkConfig, exponential_correlation
+
+
+@dataclass
+class MassiveConfig:
+    num_rx: int = 64
+    num_users: int = 8
+    num_subcarriers: int = 4096
+    active: int = 3276
+    words: int = 14
+    prefix: int = 288
+    modulation_order: int = 256
+    bandwidth: float = 4096 * 30e3
+    delays: np.ndarray = field(default_factory=lambda: np.arange(23) * 3e-6 / 22)
+    power_profile: np.ndarray | None = None
+    rice_factors: np.ndarray | None = None
+    doppler: float = 2 * np.pi * (120.0 / 3.6) * 3.5e9 / SPEED_OF_LIGHT
+    num_sinusoids: int = 20
+    rho_rx: float = 0.7
+    gain: float = 1.0
+    device_power: float = 1.0
+    dc_suppression: bool = True
+    snr_db: tuple = tuple(float(s) for s in range(10, 41, 3))
+    name: str = "massive-64x8-4096sc-256qam-lmmse"
+
+    def __post_init__(self) -> None:
+        self.delays = np.asarray(self.delays, dtype=np.float64)
+        if self.power_profile is None:
+            p = np.exp(-self.delays / 1e-6)
+            self.power_profile = p / p.sum()
+        if self.rice_factors is None:
+            k = np.zeros(self.delays.size)
+            k[0] = 1.0
+            self.rice_factors = k
+        self.power_profile = np.asarray(self.power_profile, dtype=np.float64)
+        self.rice_factors = np.asarray(self.rice_factors, dtype=np.float64)
+
+    @property
+    def corr_rx(self) -> np.ndarray:
+        return exponential_correlation(self.num_rx, self.rho_rx)
+
+    def link(self) -> LinkConfig:
+        """The channel parameters as a LinkConfig (delay samples, path amplitudes, noise power)."""
+
+        return LinkConfig(self.num_rx, self.num_users, self.num_subcarriers, self.words,
+                          self.prefix / self.num_subcarriers, self.modulation_order, self.bandwidth, self.delays,
+                          self.power_profile, self.rice_factors, self.doppler, self.num_sinusoids, gain=self.gain,
+                          device_power=self.device_power, dc_suppression=self.dc_suppression)
+
+    @property
+    def bits_per_drop(self) -> int:
+        return self.num_users * self.words * self.active * int(np.log2(self.modulation_order))
+
+    def noise_power(self, snr_db) -> np.ndarray:
+        return self.link().noise_power(snr_db)
+
+    def c_link(self) -> _lib.HpxMassiveLink:
+        lk = self.link()
+        return _lib.HpxMassiveLink(self.num_rx, self.num_users, self.num_subcarriers, self.active, self.words,
+                                   self.prefix, self.modulation_order, lk.num_paths, self.num_sinusoids, lk.max_delay,
+                                   int(bool(self.dc_suppression)), float(self.bandwidth), float(self.doppler),
+                                   float(self.doppler), float(self.gain), float(self.device_power))
+
+
+WORK_FIELDS = ("spatial", "coef", "abar", "det_v", "det_a", "det_l", "z", "grid", "csi")
+
+
+class MassiveBatch:
+    """Device tables and per-drop buffers for launches of up to `chunk` drops."""
+
+    def __init__(self, cfg: MassiveConfig, chunk: int = 32, device: str | torch.device = "cuda") -> None:
+        self.cfg = cfg
+        self.chunk = int(chunk)
+        self.device = torch.device(device)
+        lib = _lib.load()
+        self.c_link = cfg.c_link()
+        sizes = np.zeros(9, dtype=np.int64)
+        _lib.check(lib.hpx_massive_sizes(ctypes.byref(self.c_link), sizes.ctypes.data))
+        self.sizes = sizes
+        dev = self.device
+        self.buffers = {}
+        for name, count in zip(WORK_FIELDS, sizes):
+            dt = torch.float32 if name == "det_l" else torch.complex64
+            self.buffers[name] = torch.empty((self.chunk, int(count)), dtype=dt, device=dev)
+        self.work = _lib.HpxMassiveWork(*[self.buffers[n].data_ptr() for n in WORK_FIELDS])
+        lk = cfg.link()
+        self.delays = torch.as_tensor(lk.delay_samples, dtype=torch.int32, device=dev)
+        self.path_gains = torch.as_tensor(lk.path_gains, dtype=torch.float64, device=dev).contiguous()
+        self.corr_rx = torch.as_tensor(np.ascontiguousarray(cfg.corr_rx), device=dev)
+
+    def run(self, seed: int, first_drop: int, noise_power: torch.Tensor, errors: torch.Tensor,
+            equalized: torch.Tensor | None = None) -> None:
+        """Drops first_drop .. first_drop + len(noise_power) (at most `chunk`); errors[d] += bit errors."""
+
+        n = int(noise_power.numel())
+        if n > self.chunk:
+            raise ValueError(f"at most {self.chunk} drops per launch")
+        c = self.cfg
+        if equalized is not None and (equalized.dtype != torch.complex64 or
+                                      equalized.numel() < n * c.num_users * c.words * c.active):
+            raise ValueError("equalized must be complex64 [drops][users][words][active]")
+        _lib.check(_lib.load().hpx_massive_run(
+            ctypes.byref(self.c_link), _lib.ptr(self.delays), _lib.ptr(self.path_gains), _lib.ptr(self.corr_rx), None,
+            ctypes.c_uint64(seed & (2**64 - 1)), int(first_drop), n, _lib.ptr(noise_power), ctypes.byref(self.work),
+            _lib.ptr(errors), _lib.ptr(equalized), _lib.stream_handle()))

@@ -282,8 +282,8 @@ int hpx_run_batch_frames(const hpx_link* link_host, const hpx_frame* frame_host,
 
 /* out[s][p][co] = bias[co] + sum_{k, ci} weight[co][k][ci] in[s][p + k - 2][ci] on the MFMA units
  * (float32 accumulation).  mode 0: out_f32 = that, out_bf16 = bf16(that); mode 1 (GELU, tanh form):
- * out_f32 = the pre-activation, out_bf16 = bf16(gelu(pre)).  bias may be NULL; either output may be
- * NULL.  cin in {8, 64, 224}, cout a multiple of 32. */
+ * out_f32 = the pre-activation, out_bf16 = bf16(gelu(pre)); mode 2: out_f32 += that (a skip path).
+ * bias may be NULL; either output may be NULL.  cin in {8, 64, 224}, cout a multiple of 32. */
 int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, const uint16_t* weight, const float* bias,
                        int cout, int mode, float* out_f32, uint16_t* out_bf16, void* stream);
 
@@ -317,10 +317,26 @@ int hpx_net_output(const float* out, int64_t num_samples, int words, int num_sub
 int hpx_estimate_error(const hpx_c64* est, const hpx_c64* csi, const hpx_c64* spatial, int64_t num_drops,
                        int num_rx, int num_tx, int64_t grid_elements, double* err, double* power, void* stream);
 
-/* Batch NMSE loss sum |out - H|^2 / sum |H|^2 of network outputs and its gradient grad (same layout
- * as out); norm[0] and loss[0] (float64) must be zero on entry and receive sum |H|^2 and the loss. */
+/* Training loss of network outputs against the ideal CSI: the mean over drop groups (SNR points;
+ * group[d] in [0, num_groups), or NULL for one group) of 10 log10(sum |out - H|^2 / sum |H|^2),
+ * into loss[0], and its gradient grad (same layout as out).  work: float64 [3 num_groups], the first
+ * 2 num_groups zero on entry. */
 int hpx_nmse_grad(const float* out, const hpx_c64* csi, const hpx_c64* spatial, int64_t num_drops, int num_rx,
-                  int num_tx, int words, int num_subcarriers, double* norm, float* grad, double* loss, void* stream);
+                  int num_tx, int words, int num_subcarriers, const int32_t* group, int num_groups, double* work,
+                  float* grad, double* loss, void* stream);
+
+/* The learned estimator end to end in one launch: LS at every stream's pilots of received grids
+ * grid [num_drops][num_rx][words * N] (pilot_index / pilot_values [num_tx][num_pilots] as in
+ * hpx_pilot_ls, num_pilots = 64 x pilot words), the four layers and the skip convolution with the
+ * activations in LDS, and the estimates est [num_tx][num_drops][num_rx][words * N].  weights /
+ * biases: host arrays of 5 device pointers (layers 1-4, then the skip convolution), bfloat16
+ * [cout][5][cin] with cin = 8, 64, 64, 64, 8 and float32 [cout]; cout = 64 for the hidden layers and
+ * 2 words N / 64 for the last two.  Same arithmetic as hpx_net_input + hpx_conv1d_forward (x4, then
+ * mode 2) + hpx_net_output. */
+int hpx_learned_estimate(const hpx_c64* grid, int64_t num_drops, int num_rx, int num_tx, int words,
+                         int num_subcarriers, const int32_t* pilot_index, const hpx_c64* pilot_values, int num_pilots,
+                         const uint16_t* const* weights, const float* const* biases, int cout, hpx_c64* est,
+                         void* stream);
 
 /* One Adam step (bias-corrected, step >= 1) on float32 parameters. */
 int hpx_adam_step(float* param, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1, float beta2,
@@ -367,6 +383,45 @@ int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables
 
 /* One launch per class from the device copy of the plans. */
 int hpx_uneven_run(const void* plans, const hpx_uneven_class* classes_host, int num_classes, void* stream);
+
+/* ---------------------------------------------------------------- massive MIMO
+ * Uplink drops with many receive antennas (<= 64) and single-antenna users (<= 8): beyond the
+ * reference's 10-antenna arrays (its spatial response slices a 10 x 10 block of uniforms,
+ * hermespy/channel/fading/fading.py:476-489; here the block is (max(10, Nrx), max(10, Ntx)),
+ * identical for arrays of at most 10).  OFDM grid of num_subcarriers with `active` data
+ * subcarriers centred on DC (grid columns N/2 - active/2 ..), NULL elsewhere.  LMMSE detection
+ * on ideal CSI (the MMSE extension of waveform.py:235-250) with the matched filter V^H S^H y on
+ * the MFMA units (bfloat16 hi/lo split, float32 accumulation). */
+typedef struct hpx_massive_link {
+  int32_t num_rx, num_users, num_subcarriers, active, words, prefix, modulation_order;
+  int32_t num_paths, num_sinusoids, max_delay, dc_suppression;
+  double bandwidth, doppler, los_doppler, gain, device_power;
+} hpx_massive_link;
+
+/* Caller-owned per-drop device buffers; element counts per drop from hpx_massive_sizes (in this
+ * order; det_l is float32, the rest hpx_c64). */
+typedef struct hpx_massive_work {
+  hpx_c64* spatial;  /* [nr][users] */
+  hpx_c64* coef;     /* [paths][blocks][5] Taylor coefficients of the taps */
+  hpx_c64* abar;     /* [paths][words] tap means */
+  hpx_c64* det_v;    /* [users][users] */
+  hpx_c64* det_a;    /* [users][nr] V^H S^H */
+  float* det_l;      /* [users] eigenvalues of S^H S */
+  hpx_c64* z;        /* [users][words][N] delay-line outputs over the data windows */
+  hpx_c64* grid;     /* [words][nr][active] received active subcarriers */
+  hpx_c64* csi;      /* [words][active] ideal-CSI factor c (H = S c) */
+} hpx_massive_work;
+
+/* Per-drop element counts of the nine hpx_massive_work buffers. */
+int hpx_massive_sizes(const hpx_massive_link* link_host, int64_t* sizes_host);
+
+/* num_drops (<= 65535) drops first_drop.. with noise_power[d]; errors[d] += the drop's bit errors
+ * (zero them first); equalized [drops][users][words][active] or NULL.  delays / path_gains /
+ * corr_rx / corr_tx as in hpx_run_batch (corr_tx may be NULL with corr_rx given: identity). */
+int hpx_massive_run(const hpx_massive_link* link_host, const int32_t* delays, const double* path_gains,
+                    const hpx_c128* corr_rx, const hpx_c128* corr_tx, uint64_t seed, int64_t first_drop,
+                    int64_t num_drops, const double* noise_power, const hpx_massive_work* work_host, int32_t* errors,
+                    hpx_c64* equalized, void* stream);
 
 /* Host twin of the per-drop detector hpx_run_batch builds for ideal CSI, where every
  * resource element's channel is H_k = c_k S: S^H S = V diag(lambda) V^H (cyclic
