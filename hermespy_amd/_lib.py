@@ -16,7 +16,7 @@ import torch  # noqa: F401  (must precede the library load, see module docstring
 
 LIB_PATH = Path(__file__).resolve().parent / "_native" / "libhermespy_amd.so"
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 HPX_OK = 0
 HPX_ERR_INVALID = -1
@@ -133,21 +133,21 @@ SIGNATURES = {
     "hpx_lmmse_apply": (c_int, [c_vp, c_int, c_i64, c_int, c_vp, c_i64, c_vp, c_int, c_vp, c_vp]),
     "hpx_detect_estimated": (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_int, c_int,
                                      c_vp, c_u64, c_i64, c_vp, c_vp, c_vp]),
-    "hpx_conv1d_forward": (c_int, [c_vp, c_i64, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
-    "hpx_conv1d_wgrad_workspace": (c_i64, [c_i64, c_int, c_int]),
-    "hpx_conv1d_wgrad": (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "hpx_conv1d_forward": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
+    "hpx_conv1d_wgrad_workspace": (c_i64, [c_i64, c_int, c_int, c_int]),
+    "hpx_conv1d_wgrad": (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
     "hpx_gelu_backward": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
     "hpx_to_bf16": (c_int, [c_vp, c_i64, c_vp, c_vp]),
-    "hpx_net_input": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp]),
+    "hpx_net_input": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
     "hpx_net_output": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp]),
     "hpx_estimate_error": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_i64, c_vp, c_vp, c_vp]),
     "hpx_nmse_grad": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_int, c_vp, c_vp, c_vp,
                               c_vp]),
     "hpx_adam_step": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                               ctypes.c_float, c_i64, c_vp]),
-    "hpx_conv1d_weight_copies": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
-    "hpx_learned_estimate": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_int,
-                                     c_vp, c_vp]),
+    "hpx_conv1d_weight_copies": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
+    "hpx_learned_estimate": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,
+                                     c_int, c_vp, c_vp]),
     "hpx_massive_sizes": (c_int, [ctypes.POINTER(HpxMassiveLink), c_vp]),
     "hpx_massive_run": (c_int, [ctypes.POINTER(HpxMassiveLink), c_vp, c_vp, c_vp, c_vp, c_u64, c_i64, c_i64, c_vp,
                                 ctypes.POINTER(HpxMassiveWork), c_vp, c_vp, c_vp]),

@@ -21,11 +21,14 @@ using namespace hpx;
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kPos = 64;   // pilot positions per sample
-constexpr int kHalo = 2;   // kernel 5
+constexpr int kHalo = 2;   // kernel 5 of the four layers
 constexpr int kTaps = 5;
+constexpr int kSkipTaps = 17;  // the linear skip convolution
+constexpr int kSkipHalo = kSkipTaps / 2;
 
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
@@ -53,19 +56,20 @@ __device__ __forceinline__ float gelu_grad(float x) {
 // One workgroup per sample; wave v computes rows 32 (v & 1) .. +32 of output column
 // tiles v / 2, v / 2 + 2, ...  mode 0: out_f32 = acc (+ bias), out_bf16 = bf16(that);
 // mode 1: out_f32 = pre-activation, out_bf16 = bf16(gelu(pre)); mode 2: out_f32 += acc (+ bias).
-template <int CIN>
+template <int CIN, int TAPS>
 __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t* __restrict__ in,
                                                        const uint16_t* __restrict__ w,
                                                        const float* __restrict__ bias, int cout, int mode,
                                                        float* __restrict__ out_f32, uint16_t* __restrict__ out_bf16) {
   constexpr int kStride = CIN + 4;  // padded LDS row: conflict-free 8-byte reads across a half wave
-  __shared__ __align__(16) uint16_t xs[(kPos + 2 * kHalo) * kStride];
+  constexpr int kH = TAPS / 2;
+  __shared__ __align__(16) uint16_t xs[(kPos + 2 * kH) * kStride];
   const int s = blockIdx.x;
   const int tid = threadIdx.x;
   const uint16_t* xin = in + static_cast<int64_t>(s) * kPos * CIN;
-  for (int i = tid; i < (kPos + 2 * kHalo) * CIN; i += 256) {
+  for (int i = tid; i < (kPos + 2 * kH) * CIN; i += 256) {
     const int row = i / CIN, c = i - row * CIN;
-    const int p = row - kHalo;
+    const int p = row - kH;
     xs[row * kStride + c] = (p >= 0 && p < kPos) ? xin[p * CIN + c] : static_cast<uint16_t>(0);
   }
   __syncthreads();
@@ -77,10 +81,10 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t* __restric
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
     const int co = ct * 32 + m;
-    const uint16_t* wrow = w + static_cast<int64_t>(co) * kTaps * CIN + 4 * kq;
+    const uint16_t* wrow = w + static_cast<int64_t>(co) * TAPS * CIN + 4 * kq;
     const uint16_t* arow = xs + (32 * half + m) * kStride + 4 * kq;
 #pragma unroll
-    for (int k = 0; k < kTaps; ++k) {
+    for (int k = 0; k < TAPS; ++k) {
 #pragma unroll 4
       for (int c0 = 0; c0 < CIN; c0 += 8) {
         const bf16x4 a = *reinterpret_cast<const bf16x4*>(arow + k * kStride + c0);
@@ -122,13 +126,14 @@ __global__ void __launch_bounds__(256) to_bf16_kernel(const float* __restrict__ 
 constexpr int kWgSamples = 32;
 
 __global__ void __launch_bounds__(256) wgrad_kernel(const uint16_t* __restrict__ dz, const uint16_t* __restrict__ in,
-                                                    int num_samples, int cout, int cin, float* __restrict__ part,
-                                                    float* __restrict__ part_bias) {
+                                                    int num_samples, int cout, int cin, int taps,
+                                                    float* __restrict__ part, float* __restrict__ part_bias) {
   __shared__ float dzs[kPos][33];
-  __shared__ float xs[kPos + 2 * kHalo][65];
+  __shared__ float xs[kPos][65];
   const int chunk = blockIdx.x;
   const int co0 = blockIdx.y * 32;
-  const int kcols = kTaps * cin;
+  const int kcols = taps * cin;
+  const int halo = taps / 2;
   const int kc0 = blockIdx.z * 64;
   const int tid = threadIdx.x;
   // thread: co = co0 + (tid & 31), columns kc0 + (tid >> 5) + 8 j, j = 0..7
@@ -144,19 +149,16 @@ __global__ void __launch_bounds__(256) wgrad_kernel(const uint16_t* __restrict__
       dzs[p][c] = co0 + c < cout ? bf2f(dz[(static_cast<int64_t>(s) * kPos + p) * cout + co0 + c]) : 0.0f;
     }
     // input columns kc0 .. kc0 + 63 of the im2col row: (k, ci) = divmod(col, cin)
-    for (int i = tid; i < (kPos + 2 * kHalo) * 64; i += 256) {
-      const int row = i >> 6, c = i & 63;
-      xs[row][c] = 0.0f;
-    }
-    __syncthreads();
     for (int i = tid; i < kPos * 64; i += 256) {
       const int p = i >> 6, c = i & 63;
       const int col = kc0 + c;
+      float v = 0.0f;
       if (col < kcols) {
         const int k = col / cin, ci = col - k * cin;
-        const int q = p + k - kHalo;  // input position feeding output p through tap k
-        xs[p][c] = (q >= 0 && q < kPos) ? bf2f(in[(static_cast<int64_t>(s) * kPos + q) * cin + ci]) : 0.0f;
+        const int q = p + k - halo;  // input position feeding output p through tap k
+        if (q >= 0 && q < kPos) v = bf2f(in[(static_cast<int64_t>(s) * kPos + q) * cin + ci]);
       }
+      xs[p][c] = v;
     }
     __syncthreads();
     for (int p = 0; p < kPos; ++p) {
@@ -190,8 +192,14 @@ __global__ void __launch_bounds__(256) reduce_parts_kernel(const float* __restri
 
 // ---------------------------------------------------------------- pilots in, grid out
 // in[s][p][c]: c = 2 v + {0: re, 1: im} of LS value v * 64 + p (v = pilot word), c = 2 V: +-1 stream flag
+// -log10(noise power) / 3: the SNR the receiver knows (0 dB -> 0, 30 dB -> 1 at unit signal power)
+__device__ __forceinline__ float snr_feature(const double* noise_power, int64_t d) {
+  return noise_power ? static_cast<float>(-log10(noise_power[d]) / 3.0) : 0.0f;
+}
+
 __global__ void __launch_bounds__(256) net_input_kernel(const float2* __restrict__ ls, int64_t num_samples,
                                                         int num_drops, int num_rx, int pilot_words, int cin,
+                                                        const double* __restrict__ noise_power,
                                                         uint16_t* __restrict__ x) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= num_samples * kPos) return;
@@ -206,7 +214,8 @@ __global__ void __launch_bounds__(256) net_input_kernel(const float2* __restrict
     o[2 * v + 1] = f2bf(h.y);
   }
   o[2 * pilot_words] = f2bf(t ? 1.0f : -1.0f);
-  for (int c = 2 * pilot_words + 1; c < cin; ++c) o[c] = 0;
+  o[2 * pilot_words + 1] = f2bf(snr_feature(noise_power, (s % (static_cast<int64_t>(num_drops) * num_rx)) / num_rx));
+  for (int c = 2 * pilot_words + 2; c < cin; ++c) o[c] = 0;
 }
 
 // out[s][p][c], c = (w * B + u) * 2 + {re, im} -> est[s][w][B p + u]  (B = N / 64 subcarriers per position)
@@ -368,61 +377,73 @@ __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ p, const 
   p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
-__global__ void __launch_bounds__(256) weight_copies_kernel(const float* __restrict__ p, int cout, int cin,
+__global__ void __launch_bounds__(256) weight_copies_kernel(const float* __restrict__ p, int cout, int cin, int taps,
                                                             uint16_t* __restrict__ w, uint16_t* __restrict__ wt) {
-  const int64_t n = static_cast<int64_t>(cout) * kTaps * cin;
+  const int64_t n = static_cast<int64_t>(cout) * taps * cin;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int co = static_cast<int>(i / (kTaps * cin));
-  const int rest = static_cast<int>(i - static_cast<int64_t>(co) * kTaps * cin);
+  const int co = static_cast<int>(i / (taps * cin));
+  const int rest = static_cast<int>(i - static_cast<int64_t>(co) * taps * cin);
   const int k = rest / cin, ci = rest - k * cin;
   const uint16_t v = f2bf(p[i]);
   if (w) w[i] = v;
-  if (wt) wt[(static_cast<int64_t>(ci) * kTaps + (kTaps - 1 - k)) * cout + co] = v;
+  if (wt) wt[(static_cast<int64_t>(ci) * taps + (taps - 1 - k)) * cout + co] = v;
 }
 
 // ---------------------------------------------------------------- fused inference
 // The whole estimator for G samples per workgroup: LS at the sample's pilots (y_p / x_p), the four
 // layers and the skip convolution, with every activation in LDS (bfloat16, zero halo rows), and
 // the output written straight into the estimate grid.  Numerically the layer-by-layer path: the
-// same float32 accumulation, bias, GELU and bfloat16 rounding between layers.
+// same float32 accumulation, bias, GELU and bfloat16 rounding between layers; the 64-channel
+// layers use the 32x32x16 MFMA (8 channels per lane and step).
 constexpr int kFusedSamples = 2;
-constexpr int kS8 = 8 + 4, kS64 = 64 + 4;  // padded LDS rows (8-byte reads conflict-free per half wave)
+constexpr int kS8 = 8 + 4, kS64 = 64 + 8;  // padded LDS rows
+constexpr int kXRows = kPos + 2 * kSkipHalo;   // input rows (halo of the 17-tap skip)
+constexpr int kARows = kPos + 2 * kHalo;       // hidden activation rows
 
-template <int CIN, int STRIDE_IN>
-__device__ __forceinline__ void fused_tile(const uint16_t (*in)[(kPos + 2 * kHalo) * STRIDE_IN],
-                                           const uint16_t* __restrict__ w, int co, int half, int m, int kq,
-                                           floatx16 (&acc)[kFusedSamples]) {
-  const uint16_t* wrow = w + static_cast<int64_t>(co) * kTaps * CIN + 4 * kq;
+// 8-channel input (32x32x8 steps): taps k = 0..TAPS-1 read rows ROW0 + p + k
+template <int TAPS, int ROW0>
+__device__ __forceinline__ void fused_tile8(const uint16_t (*in)[kXRows * kS8], const uint16_t* __restrict__ w, int co,
+                                            int half, int m, int kq, floatx16 (&acc)[kFusedSamples]) {
+  const uint16_t* wrow = w + static_cast<int64_t>(co) * TAPS * 8 + 4 * kq;
+#pragma unroll 1
+  for (int k = 0; k < TAPS; ++k) {
+    const bf16x4 b = *reinterpret_cast<const bf16x4*>(wrow + k * 8);
 #pragma unroll
+    for (int g = 0; g < kFusedSamples; ++g) {
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(&in[g][(ROW0 + 32 * half + m + k) * kS8 + 4 * kq]);
+      acc[g] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[g], 0, 0, 0);
+    }
+  }
+}
+
+// 64-channel input (32x32x16 steps: lanes 0-31 take channels c0..c0+7, lanes 32-63 c0+8..c0+15)
+__device__ __forceinline__ void fused_tile64(const uint16_t (*in)[kARows * kS64], const uint16_t* __restrict__ w,
+                                             int co, int half, int m, int kq, floatx16 (&acc)[kFusedSamples]) {
+  const uint16_t* wrow = w + static_cast<int64_t>(co) * kTaps * 64 + 8 * kq;
+#pragma unroll 1
   for (int k = 0; k < kTaps; ++k) {
-#pragma unroll 2
-    for (int c0 = 0; c0 < CIN; c0 += 8) {
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(wrow + k * CIN + c0);
+#pragma unroll
+    for (int c0 = 0; c0 < 64; c0 += 16) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(wrow + k * 64 + c0);
 #pragma unroll
       for (int g = 0; g < kFusedSamples; ++g) {
-        const bf16x4 a =
-            *reinterpret_cast<const bf16x4*>(&in[g][(32 * half + m + k) * STRIDE_IN + 4 * kq + c0]);
-        acc[g] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[g], 0, 0, 0);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&in[g][(32 * half + m + k) * kS64 + 8 * kq + c0]);
+        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[g], 0, 0, 0);
       }
     }
   }
 }
 
-template <int CIN, int STRIDE_IN>
-__device__ __forceinline__ void fused_hidden(const uint16_t (*in)[(kPos + 2 * kHalo) * STRIDE_IN],
-                                             uint16_t (*out)[(kPos + 2 * kHalo) * kS64],
-                                             const uint16_t* __restrict__ w, const float* __restrict__ bias) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int half = wave & 1, ct = wave >> 1;  // 64 outputs = 2 column tiles x 2 row halves
-  const int m = lane & 31, kq = lane >> 5;
-  const int co = ct * 32 + m;
-  floatx16 acc[kFusedSamples];
+__device__ __forceinline__ void zero_acc(floatx16 (&acc)[kFusedSamples]) {
 #pragma unroll
   for (int g = 0; g < kFusedSamples; ++g)
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[g][r] = 0.0f;
-  fused_tile<CIN, STRIDE_IN>(in, w, co, half, m, kq, acc);
+}
+
+__device__ __forceinline__ void store_hidden(const floatx16 (&acc)[kFusedSamples], const float* __restrict__ bias,
+                                             int co, int half, int kq, uint16_t (*out)[kARows * kS64]) {
   const float bv = bias[co];
 #pragma unroll
   for (int g = 0; g < kFusedSamples; ++g)
@@ -433,35 +454,40 @@ __device__ __forceinline__ void fused_hidden(const uint16_t (*in)[(kPos + 2 * kH
     }
 }
 
-__global__ void __launch_bounds__(256) learned_estimate_kernel(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) learned_estimate_kernel(
     const float2* __restrict__ grid, int num_drops, int num_rx, int words, int n, const int32_t* __restrict__ pidx,
-    const float2* __restrict__ pval, int num_pilots, int64_t num_samples, const uint16_t* __restrict__ w1,
-    const float* __restrict__ b1, const uint16_t* __restrict__ w2, const float* __restrict__ b2,
-    const uint16_t* __restrict__ w3, const float* __restrict__ b3, const uint16_t* __restrict__ w4,
-    const float* __restrict__ b4, const uint16_t* __restrict__ wsk, const float* __restrict__ bsk, int cout,
-    float* __restrict__ est) {
-  __shared__ __align__(16) uint16_t xs[kFusedSamples][(kPos + 2 * kHalo) * kS8];
-  __shared__ __align__(16) uint16_t a0[kFusedSamples][(kPos + 2 * kHalo) * kS64];
-  __shared__ __align__(16) uint16_t a1[kFusedSamples][(kPos + 2 * kHalo) * kS64];
+    const float2* __restrict__ pval, int num_pilots, const double* __restrict__ noise_power, int64_t num_samples,
+    const uint16_t* __restrict__ w1, const float* __restrict__ b1, const uint16_t* __restrict__ w2,
+    const float* __restrict__ b2, const uint16_t* __restrict__ w3, const float* __restrict__ b3,
+    const uint16_t* __restrict__ w4, const float* __restrict__ b4, const uint16_t* __restrict__ wsk,
+    const float* __restrict__ bsk, int cout, float* __restrict__ est) {
+  __shared__ __align__(16) uint16_t xs[kFusedSamples][kXRows * kS8];
+  __shared__ __align__(16) uint16_t a0[kFusedSamples][kARows * kS64];
+  __shared__ __align__(16) uint16_t a1[kFusedSamples][kARows * kS64];
   const int tid = threadIdx.x;
   const int64_t s0 = static_cast<int64_t>(blockIdx.x) * kFusedSamples;
   const int64_t wn = static_cast<int64_t>(words) * n;
   const int64_t dr_count = static_cast<int64_t>(num_drops) * num_rx;
   const int pilot_words = num_pilots / kPos;
-  // zero halo rows of every buffer
+  // zero halo rows
   for (int i = tid; i < kFusedSamples * 2 * kHalo * kS64; i += 256) {
     const int g = i / (2 * kHalo * kS64), rest = i - g * 2 * kHalo * kS64;
     const int row = rest / kS64, c = rest - row * kS64;
-    const int rr = row < kHalo ? row : kPos + row;  // rows 0, 1 and 66, 67
+    const int rr = row < kHalo ? row : kPos + row;
     a0[g][rr * kS64 + c] = 0;
     a1[g][rr * kS64 + c] = 0;
-    if (c < kS8) xs[g][rr * kS8 + c] = 0;
   }
-  // input: LS at the sample's own pilots, two channels per pilot word, then the stream flag
+  for (int i = tid; i < kFusedSamples * 2 * kSkipHalo * kS8; i += 256) {
+    const int g = i / (2 * kSkipHalo * kS8), rest = i - g * 2 * kSkipHalo * kS8;
+    const int row = rest / kS8, c = rest - row * kS8;
+    const int rr = row < kSkipHalo ? row : kPos + row;
+    xs[g][rr * kS8 + c] = 0;
+  }
+  // input: LS at the sample's own pilots, two channels per pilot word, the stream flag, the SNR
   for (int i = tid; i < kFusedSamples * kPos; i += 256) {
     const int g = i / kPos, p = i - g * kPos;
     const int64_t s = s0 + g;
-    uint16_t* row = &xs[g][(p + kHalo) * kS8];
+    uint16_t* row = &xs[g][(p + kSkipHalo) * kS8];
     if (s >= num_samples) {
       for (int c = 0; c < 8; ++c) row[c] = 0;
       continue;
@@ -477,35 +503,45 @@ __global__ void __launch_bounds__(256) learned_estimate_kernel(
       row[2 * v + 1] = f2bf(h.y);
     }
     row[2 * pilot_words] = f2bf(t ? 1.0f : -1.0f);
-    for (int c = 2 * pilot_words + 1; c < 8; ++c) row[c] = 0;
+    row[2 * pilot_words + 1] = f2bf(snr_feature(noise_power, dr / num_rx));
+    for (int c = 2 * pilot_words + 2; c < 8; ++c) row[c] = 0;
   }
   __syncthreads();
-  fused_hidden<8, kS8>(xs, a0, w1, b1);
-  __syncthreads();
-  fused_hidden<64, kS64>(a0, a1, w2, b2);
-  __syncthreads();
-  fused_hidden<64, kS64>(a1, a0, w3, b3);
-  __syncthreads();
-  // last layer + skip convolution straight into the estimate grid:
-  // channel co = 2 (w B + u) + {re, im} of position p -> est[s][w][B p + u]
   const int wave = tid >> 6, lane = tid & 63;
   const int half = wave & 1;
   const int m = lane & 31, kq = lane >> 5;
+  floatx16 acc[kFusedSamples];
+  {  // layer 1: 8 -> 64 (column tile wave / 2)
+    const int co = (wave >> 1) * 32 + m;
+    zero_acc(acc);
+    fused_tile8<kTaps, kSkipHalo - kHalo>(xs, w1, co, half, m, kq, acc);
+    store_hidden(acc, b1, co, half, kq, a0);
+  }
+  __syncthreads();
+  {
+    const int co = (wave >> 1) * 32 + m;
+    zero_acc(acc);
+    fused_tile64(a0, w2, co, half, m, kq, acc);
+    store_hidden(acc, b2, co, half, kq, a1);
+  }
+  __syncthreads();
+  {
+    const int co = (wave >> 1) * 32 + m;
+    zero_acc(acc);
+    fused_tile64(a1, w3, co, half, m, kq, acc);
+    store_hidden(acc, b3, co, half, kq, a0);
+  }
+  __syncthreads();
+  // last layer + skip convolution straight into the estimate grid:
+  // channel co = 2 (w B + u) + {re, im} of position p -> est[s][w][B p + u]
   const int blk = n / kPos;
   for (int ct = wave >> 1; ct < cout / 32; ct += 2) {
     const int co = ct * 32 + m;
-    floatx16 acc[kFusedSamples];
-#pragma unroll
-    for (int g = 0; g < kFusedSamples; ++g)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[g][r] = 0.0f;
-    fused_tile<64, kS64>(a0, w4, co, half, m, kq, acc);
+    zero_acc(acc);
+    fused_tile64(a0, w4, co, half, m, kq, acc);
     floatx16 sk[kFusedSamples];
-#pragma unroll
-    for (int g = 0; g < kFusedSamples; ++g)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) sk[g][r] = 0.0f;
-    fused_tile<8, kS8>(xs, wsk, co, half, m, kq, sk);
+    zero_acc(sk);
+    fused_tile8<kSkipTaps, 0>(xs, wsk, co, half, m, kq, sk);
     const float bv = b4[co], bs = bsk[co];
     const int c2 = co >> 1, reim = co & 1;
     const int w = c2 / blk, u = c2 - w * blk;
@@ -526,7 +562,7 @@ unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); 
 
 }  // namespace
 
-extern "C" int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, const uint16_t* weight,
+extern "C" int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, int taps, const uint16_t* weight,
                                   const float* bias, int cout, int mode, float* out_f32, uint16_t* out_bf16,
                                   void* stream) {
   HPX_REQUIRE(num_samples >= 0 && num_samples <= 0x7fffffff, "hpx_conv1d_forward: invalid sample count");
@@ -536,35 +572,40 @@ extern "C" int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int c
   HPX_REQUIRE(in && weight && (out_f32 || out_bf16) && (mode != 2 || out_f32), "hpx_conv1d_forward: null pointer");
   const dim3 g(static_cast<unsigned>(num_samples));
   const hipStream_t s = as_stream(stream);
-  switch (cin) {
-    case 8: hipLaunchKernelGGL(conv_fwd_kernel<8>, g, dim3(256), 0, s, in, weight, bias, cout, mode, out_f32, out_bf16); break;
-    case 64: hipLaunchKernelGGL(conv_fwd_kernel<64>, g, dim3(256), 0, s, in, weight, bias, cout, mode, out_f32, out_bf16); break;
-    case 224: hipLaunchKernelGGL(conv_fwd_kernel<224>, g, dim3(256), 0, s, in, weight, bias, cout, mode, out_f32, out_bf16); break;
-    default:
-      set_error("hpx_conv1d_forward: cin %d not instantiated (8, 64, 224)", cin);
-      return HPX_ERR_UNSUPPORTED;
+#define HPX_CONV(CIN, TAPS) \
+  hipLaunchKernelGGL((conv_fwd_kernel<CIN, TAPS>), g, dim3(256), 0, s, in, weight, bias, cout, mode, out_f32, out_bf16)
+  if (cin == 8 && taps == kTaps) HPX_CONV(8, kTaps);
+  else if (cin == 8 && taps == kSkipTaps) HPX_CONV(8, kSkipTaps);
+  else if (cin == 64 && taps == kTaps) HPX_CONV(64, kTaps);
+  else if (cin == 224 && taps == kTaps) HPX_CONV(224, kTaps);
+  else {
+    set_error("hpx_conv1d_forward: (cin %d, taps %d) not instantiated ((8|64|224, 5), (8, 17))", cin, taps);
+    return HPX_ERR_UNSUPPORTED;
   }
+#undef HPX_CONV
   return check_launch("hpx_conv1d_forward");
 }
 
-extern "C" int64_t hpx_conv1d_wgrad_workspace(int64_t num_samples, int cin, int cout) {
+extern "C" int64_t hpx_conv1d_wgrad_workspace(int64_t num_samples, int cin, int cout, int taps) {
   const int64_t chunks = (num_samples + kWgSamples - 1) / kWgSamples;
-  return chunks * static_cast<int64_t>(cout) * (kTaps * cin + 1) * static_cast<int64_t>(sizeof(float));
+  return chunks * static_cast<int64_t>(cout) * (static_cast<int64_t>(taps) * cin + 1) *
+         static_cast<int64_t>(sizeof(float));
 }
 
 extern "C" int hpx_conv1d_wgrad(const uint16_t* dz, const uint16_t* in, int64_t num_samples, int cin, int cout,
-                                float* workspace, float* grad_weight, float* grad_bias, void* stream) {
-  HPX_REQUIRE(num_samples >= 1 && num_samples <= 0x7fffffff && cin >= 1 && cout >= 1,
+                                int taps, float* workspace, float* grad_weight, float* grad_bias, void* stream) {
+  HPX_REQUIRE(num_samples >= 1 && num_samples <= 0x7fffffff && cin >= 1 && cout >= 1 && taps >= 1 && (taps & 1) &&
+                  taps < kPos,
               "hpx_conv1d_wgrad: invalid sizes");
   HPX_REQUIRE(dz && in && workspace && grad_weight && grad_bias, "hpx_conv1d_wgrad: null pointer");
   const int chunks = static_cast<int>((num_samples + kWgSamples - 1) / kWgSamples);
-  const int64_t nw = static_cast<int64_t>(cout) * kTaps * cin;
+  const int64_t nw = static_cast<int64_t>(cout) * taps * cin;
   float* part_w = workspace;
   float* part_b = workspace + static_cast<int64_t>(chunks) * nw;
   const hipStream_t s = as_stream(stream);
   const dim3 g(static_cast<unsigned>(chunks), static_cast<unsigned>((cout + 31) / 32),
-               static_cast<unsigned>((kTaps * cin + 63) / 64));
-  hipLaunchKernelGGL(wgrad_kernel, g, dim3(256), 0, s, dz, in, static_cast<int>(num_samples), cout, cin, part_w,
+               static_cast<unsigned>((taps * cin + 63) / 64));
+  hipLaunchKernelGGL(wgrad_kernel, g, dim3(256), 0, s, dz, in, static_cast<int>(num_samples), cout, cin, taps, part_w,
                      part_b);
   hipLaunchKernelGGL(reduce_parts_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, part_w, chunks, nw, grad_weight);
   hipLaunchKernelGGL(reduce_parts_kernel, dim3(blocks_for(cout)), dim3(256), 0, s, part_b, chunks,
@@ -589,16 +630,16 @@ extern "C" int hpx_to_bf16(const float* x, int64_t n, uint16_t* y, void* stream)
 }
 
 extern "C" int hpx_net_input(const hpx_c64* ls, int64_t num_drops, int num_rx, int num_tx, int pilot_words, int cin,
-                             uint16_t* x, void* stream) {
+                             const double* noise_power, uint16_t* x, void* stream) {
   HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff && num_rx >= 1 && num_tx >= 1 && pilot_words >= 1 &&
-                  cin >= 2 * pilot_words + 1,
+                  cin >= 2 * pilot_words + 2,
               "hpx_net_input: invalid sizes");
   const int64_t samples = static_cast<int64_t>(num_tx) * num_drops * num_rx;
   if (samples == 0) return HPX_OK;
   HPX_REQUIRE(ls && x, "hpx_net_input: null pointer");
   hipLaunchKernelGGL(net_input_kernel, dim3(blocks_for(samples * kPos)), dim3(256), 0, as_stream(stream),
                      reinterpret_cast<const float2*>(ls), samples, static_cast<int>(num_drops), num_rx, pilot_words,
-                     cin, x);
+                     cin, noise_power, x);
   return check_launch("hpx_net_input");
 }
 
@@ -665,24 +706,24 @@ extern "C" int hpx_adam_step(float* param, const float* grad, float* m1, float* 
   return check_launch("hpx_adam_step");
 }
 
-extern "C" int hpx_conv1d_weight_copies(const float* param, int cout, int cin, uint16_t* weight,
+extern "C" int hpx_conv1d_weight_copies(const float* param, int cout, int cin, int taps, uint16_t* weight,
                                         uint16_t* weight_transposed, void* stream) {
-  HPX_REQUIRE(cout >= 1 && cin >= 1, "hpx_conv1d_weight_copies: invalid sizes");
+  HPX_REQUIRE(cout >= 1 && cin >= 1 && taps >= 1, "hpx_conv1d_weight_copies: invalid sizes");
   HPX_REQUIRE(param && (weight || weight_transposed), "hpx_conv1d_weight_copies: null pointer");
-  const int64_t n = static_cast<int64_t>(cout) * kTaps * cin;
+  const int64_t n = static_cast<int64_t>(cout) * taps * cin;
   hipLaunchKernelGGL(weight_copies_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), param, cout, cin,
-                     weight, weight_transposed);
+                     taps, weight, weight_transposed);
   return check_launch("hpx_conv1d_weight_copies");
 }
 
 extern "C" int hpx_learned_estimate(const hpx_c64* grid, int64_t num_drops, int num_rx, int num_tx, int words,
                                     int num_subcarriers, const int32_t* pilot_index, const hpx_c64* pilot_values,
-                                    int num_pilots, const uint16_t* const* weights, const float* const* biases,
-                                    int cout, hpx_c64* est, void* stream) {
+                                    int num_pilots, const double* noise_power, const uint16_t* const* weights,
+                                    const float* const* biases, int cout, hpx_c64* est, void* stream) {
   HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff && num_rx >= 1 && num_tx >= 1 && words >= 1,
               "hpx_learned_estimate: invalid sizes");
   HPX_REQUIRE(num_subcarriers >= kPos && num_subcarriers % kPos == 0 && num_pilots >= kPos && num_pilots % kPos == 0 &&
-                  2 * (num_pilots / kPos) + 1 <= 8,
+                  2 * (num_pilots / kPos) + 2 <= 8,
               "hpx_learned_estimate: needs 64-position pilot words (at most 3) and a multiple of 64 subcarriers");
   HPX_REQUIRE(cout == 2 * words * (num_subcarriers / kPos) && cout % 32 == 0,
               "hpx_learned_estimate: output channels %d do not match the grid", cout);
@@ -694,8 +735,8 @@ extern "C" int hpx_learned_estimate(const hpx_c64* grid, int64_t num_drops, int 
   HPX_REQUIRE(blocks <= 0x7fffffff, "hpx_learned_estimate: too many samples");
   hipLaunchKernelGGL(learned_estimate_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, as_stream(stream),
                      reinterpret_cast<const float2*>(grid), static_cast<int>(num_drops), num_rx, words,
-                     num_subcarriers, pilot_index, reinterpret_cast<const float2*>(pilot_values), num_pilots, samples,
-                     weights[0], biases[0], weights[1], biases[1], weights[2], biases[2], weights[3], biases[3],
+                     num_subcarriers, pilot_index, reinterpret_cast<const float2*>(pilot_values), num_pilots, noise_power,
+                     samples, weights[0], biases[0], weights[1], biases[1], weights[2], biases[2], weights[3], biases[3],
                      weights[4], biases[4], cout, reinterpret_cast<float*>(est));
   return check_launch("hpx_learned_estimate");
 }

@@ -2,11 +2,11 @@
 
 A 1-D convolutional network over the 64 pilot positions of one (receive,
 transmit) stream pair maps the LS values at that stream's pilots (two pilot
-words, real and imaginary parts, plus a stream flag: 5 of 8 input channels) to
+words, real and imaginary parts, a stream flag and the SNR: 6 of 8 input channels) to
 the channel of every resource element: 64 positions x (words x N / 64
 subcarriers x re/im) outputs.  Four layers, kernel 5, 64 hidden channels, GELU
-(tanh form), plus a linear kernel-5 skip convolution from the input to the
-output (the local linear interpolator the nonlinear path refines): about 125k
+(tanh form), plus a linear 17-tap skip convolution from the input to the
+output (the linear interpolator the nonlinear path refines): about 146k
 parameters.  Weights and activations are bfloat16,
 products accumulate in float32 on the MFMA units (``hpx_conv1d_forward``);
 training runs on the device with hand-written kernels: backward-data as the
@@ -33,8 +33,9 @@ from .configs import LinkConfig
 
 POSITIONS = 64
 TAPS = 5
+SKIP_TAPS = 17
 HIDDEN = 64
-CIN = 8  # 2 pilot words x (re, im) + stream flag, zero-padded to 8
+CIN = 8  # 2 pilot words x (re, im) + stream flag + SNR, zero-padded to 8
 WEIGHTS_PATH = Path(__file__).resolve().parent / "data" / "learned_estimator.npz"
 
 
@@ -46,7 +47,7 @@ def layer_shapes(cfg: LinkConfig) -> list:
 
 
 def skip_shape(cfg: LinkConfig) -> tuple:
-    """(cin, cout) of the linear skip convolution."""
+    """(cin, cout) of the linear skip convolution (SKIP_TAPS taps)."""
 
     return CIN, layer_shapes(cfg)[-1][1]
 
@@ -63,7 +64,7 @@ def init_params(cfg: LinkConfig, seed: int = 1) -> list:
         b = rng.uniform(-bound, bound, cout).astype(np.float32)
         out.append((w, b))
     cin, cout = skip_shape(cfg)
-    out.append((np.zeros((cout, TAPS, cin), np.float32), np.zeros(cout, np.float32)))
+    out.append((np.zeros((cout, SKIP_TAPS, cin), np.float32), np.zeros(cout, np.float32)))
     return out
 
 
@@ -89,7 +90,7 @@ class ConvNetEstimator:
         self.w = [torch.as_tensor(np.ascontiguousarray(w, dtype=np.float32), device=dev) for w, _ in params]
         self.b = [torch.as_tensor(np.ascontiguousarray(b, dtype=np.float32), device=dev) for _, b in params]
         self.w16 = [torch.empty(w.shape, dtype=torch.bfloat16, device=dev) for w in self.w]
-        self.wt16 = [torch.empty((w.shape[2], TAPS, w.shape[0]), dtype=torch.bfloat16, device=dev) for w in self.w]
+        self.wt16 = [torch.empty((w.shape[2], w.shape[1], w.shape[0]), dtype=torch.bfloat16, device=dev) for w in self.w]
         self.refresh()
 
     @property
@@ -101,19 +102,19 @@ class ConvNetEstimator:
 
         lib, st = _lib.load(), _lib.stream_handle()
         for w, w16, wt16 in zip(self.w, self.w16, self.wt16):
-            _lib.check(lib.hpx_conv1d_weight_copies(_lib.ptr(w), w.shape[0], w.shape[2], _lib.ptr(w16), _lib.ptr(wt16),
-                                                    st))
+            _lib.check(lib.hpx_conv1d_weight_copies(_lib.ptr(w), w.shape[0], w.shape[2], w.shape[1], _lib.ptr(w16),
+                                                    _lib.ptr(wt16), st))
 
     # ------------------------------------------------------------ inference
-    def net_input(self, ls: torch.Tensor) -> torch.Tensor:
-        """LS values [num_tx][drops][num_rx][pilot_words * 64] -> bf16 input [S][64][8]."""
+    def net_input(self, ls: torch.Tensor, noise_power: torch.Tensor) -> torch.Tensor:
+        """LS values [num_tx][drops][num_rx][pilot_words * 64] and the drops' noise powers -> bf16 input
+        [S][64][8]."""
 
-        c = self.cfg
         num_tx, drops, num_rx, p = ls.shape
         words = p // POSITIONS
         x = torch.empty((num_tx * drops * num_rx, POSITIONS, CIN), dtype=torch.bfloat16, device=self.device)
-        _lib.check(_lib.load().hpx_net_input(_lib.ptr(ls), drops, num_rx, num_tx, words, CIN, _lib.ptr(x),
-                                             _lib.stream_handle()))
+        _lib.check(_lib.load().hpx_net_input(_lib.ptr(ls), drops, num_rx, num_tx, words, CIN, _lib.ptr(noise_power),
+                                             _lib.ptr(x), _lib.stream_handle()))
         return x
 
     def forward(self, x: torch.Tensor, keep: bool = False):
@@ -128,19 +129,19 @@ class ConvNetEstimator:
             last = i == len(self.shapes) - 1
             pre = torch.empty((s, POSITIONS, cout), dtype=torch.float32, device=self.device) if (keep or last) else None
             nxt = None if last else torch.empty((s, POSITIONS, cout), dtype=torch.bfloat16, device=self.device)
-            _lib.check(lib.hpx_conv1d_forward(_lib.ptr(a), s, cin, _lib.ptr(self.w16[i]), _lib.ptr(self.b[i]), cout,
-                                              1 if act else 0, _lib.ptr(pre), _lib.ptr(nxt), st))
+            _lib.check(lib.hpx_conv1d_forward(_lib.ptr(a), s, cin, TAPS, _lib.ptr(self.w16[i]), _lib.ptr(self.b[i]),
+                                              cout, 1 if act else 0, _lib.ptr(pre), _lib.ptr(nxt), st))
             cache.append((a, pre))
             a = nxt
             out = pre
         # linear skip path from the input, accumulated onto the last layer's output
         cin, cout = skip_shape(self.cfg)
-        _lib.check(lib.hpx_conv1d_forward(_lib.ptr(x), s, cin, _lib.ptr(self.w16[-1]), _lib.ptr(self.b[-1]), cout, 2,
-                                          _lib.ptr(out), None, st))
+        _lib.check(lib.hpx_conv1d_forward(_lib.ptr(x), s, cin, SKIP_TAPS, _lib.ptr(self.w16[-1]), _lib.ptr(self.b[-1]),
+                                          cout, 2, _lib.ptr(out), None, st))
         return (out, cache) if keep else out
 
     def estimate_grid(self, grid: torch.Tensor, pilot_index: torch.Tensor, pilot_values: torch.Tensor,
-                      out: torch.Tensor | None = None) -> torch.Tensor:
+                      noise_power: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Channel estimates [num_tx][drops][num_rx][words * N] of received grids [drops][num_rx][words][N]
         in one launch (``hpx_learned_estimate``: pilot LS, the network and the skip path with the
         activations in LDS)."""
@@ -156,17 +157,17 @@ class ConvNetEstimator:
         bs = (ctypes.c_void_p * 5)(*[b.data_ptr() for b in self.b])
         _lib.check(_lib.load().hpx_learned_estimate(
             _lib.ptr(grid), drops, c.num_rx, c.num_tx, c.words, c.num_subcarriers, _lib.ptr(pilot_index),
-            _lib.ptr(pilot_values), pilot_index.shape[1], ctypes.addressof(ws), ctypes.addressof(bs),
+            _lib.ptr(pilot_values), pilot_index.shape[1], _lib.ptr(noise_power), ctypes.addressof(ws), ctypes.addressof(bs),
             self.shapes[-1][1], _lib.ptr(out), _lib.stream_handle()))
         return out
 
-    def estimate(self, ls: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    def estimate(self, ls: torch.Tensor, noise_power: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Channel estimates [num_tx][drops][num_rx][words * N] from LS values (layer by layer, the
         training path's arithmetic)."""
 
         c = self.cfg
         num_tx, drops, num_rx, _ = ls.shape
-        y = self.forward(self.net_input(ls))
+        y = self.forward(self.net_input(ls, noise_power))
         if out is None:
             out = torch.empty((num_tx, drops, num_rx, c.words * c.num_subcarriers), dtype=torch.complex64,
                               device=self.device)
@@ -186,6 +187,8 @@ class ConvNetEstimator:
     def load(cls, cfg: LinkConfig, path: str | Path = WEIGHTS_PATH, device: str | torch.device = "cuda"):
         with np.load(path, allow_pickle=False) as f:
             params = [(f[f"w{i}"], f[f"b{i}"]) for i in range(5)]
+        if params[-1][0].shape[1] != SKIP_TAPS:
+            raise ValueError(f"{path}: skip convolution with {params[-1][0].shape[1]} taps, expected {SKIP_TAPS}")
         return cls(cfg, params, device)
 
 
@@ -218,7 +221,8 @@ class Trainer:
         self.gb = [torch.empty_like(b) for b in net.b]
         s = self.cfg.num_tx * drops_per_step * self.cfg.num_rx
         lib = _lib.load()
-        ws = max(int(lib.hpx_conv1d_wgrad_workspace(s, cin, cout)) for cin, cout, _ in net.shapes)
+        ws = max([int(lib.hpx_conv1d_wgrad_workspace(s, cin, cout, TAPS)) for cin, cout, _ in net.shapes] +
+                 [int(lib.hpx_conv1d_wgrad_workspace(s, *skip_shape(self.cfg), SKIP_TAPS))])
         self.workspace = torch.empty(ws // 4 + 1, dtype=torch.float32, device=dev)
         c = self.cfg
         self.grid = torch.empty((drops_per_step, c.num_rx, c.words, c.num_subcarriers), dtype=torch.complex64,
@@ -238,6 +242,7 @@ class Trainer:
         snr = np.asarray(c.snr_db)
         npw = torch.as_tensor(c.noise_power(snr[(first_drop + np.arange(n)) % snr.size]), dtype=torch.float64,
                               device=self.net.device)
+        self.noise_power = npw
         self.link.run(self.seed, first_drop, npw, self.errors, doppler=c.draw_doppler(first_drop, n), grid=self.grid,
                       csi=self.csi, spatial=self.spatial)
         ls = torch.empty((c.num_tx, n, c.num_rx, self.num_pilots), dtype=torch.complex64, device=self.net.device)
@@ -253,7 +258,7 @@ class Trainer:
         net, c = self.net, self.cfg
         lib, st = _lib.load(), _lib.stream_handle()
         ls = self.simulate(first_drop)
-        x = net.net_input(ls)
+        x = net.net_input(ls, self.noise_power)
         out, cache = net.forward(x, keep=True)
         grad = torch.empty_like(out)
         group = torch.as_tensor(((first_drop + np.arange(self.drops)) % self.num_groups).astype(np.int32),
@@ -271,15 +276,16 @@ class Trainer:
                 _lib.check(lib.hpx_gelu_backward(_lib.ptr(grad), _lib.ptr(pre), grad.numel(), _lib.ptr(dz), st))
             else:
                 _lib.check(lib.hpx_to_bf16(_lib.ptr(grad), grad.numel(), _lib.ptr(dz), st))
-            _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(a_in), s, cin, cout, _lib.ptr(self.workspace),
+            _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(a_in), s, cin, cout, TAPS, _lib.ptr(self.workspace),
                                             _lib.ptr(self.gw[i]), _lib.ptr(self.gb[i]), st))
             if i == len(net.shapes) - 1:  # the skip convolution sees the same output gradient
                 scin, scout = skip_shape(c)
-                _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(x), s, scin, scout, _lib.ptr(self.workspace),
-                                                _lib.ptr(self.gw[-1]), _lib.ptr(self.gb[-1]), st))
+                _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(x), s, scin, scout, SKIP_TAPS,
+                                                _lib.ptr(self.workspace), _lib.ptr(self.gw[-1]), _lib.ptr(self.gb[-1]),
+                                                st))
             if i > 0:
                 grad = torch.empty((s, POSITIONS, cin), dtype=torch.float32, device=net.device)
-                _lib.check(lib.hpx_conv1d_forward(_lib.ptr(dz), s, cout, _lib.ptr(net.wt16[i]), None, cin, 0,
+                _lib.check(lib.hpx_conv1d_forward(_lib.ptr(dz), s, cout, TAPS, _lib.ptr(net.wt16[i]), None, cin, 0,
                                                   _lib.ptr(grad), None, st))
         self.step_count += 1
         b1, b2 = self.betas

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define HPX_ABI_VERSION 2
+#define HPX_ABI_VERSION 3
 
 typedef enum hpx_status {
   HPX_OK = 0,
@@ -278,21 +278,24 @@ int hpx_run_batch_frames(const hpx_link* link_host, const hpx_frame* frame_host,
  * OrthogonalLeastSquaresChannelEstimation (orthogonal/channel.py:29-66); no reference
  * counterpart otherwise.  Activations are bfloat16 (uint16 bit patterns)
  * [samples][64][channels], sample = (t * num_drops + d) * num_rx + r; weights
- * bfloat16 [cout][5][cin], kernel 5 with zero padding 2. */
+ * bfloat16 [cout][taps][cin] (odd taps, zero padding taps / 2): 5 for the four
+ * layers, 17 for the linear skip convolution from the input. */
 
-/* out[s][p][co] = bias[co] + sum_{k, ci} weight[co][k][ci] in[s][p + k - 2][ci] on the MFMA units
- * (float32 accumulation).  mode 0: out_f32 = that, out_bf16 = bf16(that); mode 1 (GELU, tanh form):
- * out_f32 = the pre-activation, out_bf16 = bf16(gelu(pre)); mode 2: out_f32 += that (a skip path).
- * bias may be NULL; either output may be NULL.  cin in {8, 64, 224}, cout a multiple of 32. */
-int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, const uint16_t* weight, const float* bias,
-                       int cout, int mode, float* out_f32, uint16_t* out_bf16, void* stream);
+/* out[s][p][co] = bias[co] + sum_{k, ci} weight[co][k][ci] in[s][p + k - taps/2][ci] on the MFMA
+ * units (float32 accumulation).  mode 0: out_f32 = that, out_bf16 = bf16(that); mode 1 (GELU, tanh
+ * form): out_f32 = the pre-activation, out_bf16 = bf16(gelu(pre)); mode 2: out_f32 += that (a skip
+ * path).  bias may be NULL; either output may be NULL.  (cin, taps) in {(8|64|224, 5), (8, 17)},
+ * cout a multiple of 32. */
+int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, int taps, const uint16_t* weight,
+                       const float* bias, int cout, int mode, float* out_f32, uint16_t* out_bf16, void* stream);
 
 /* float32 workspace bytes of hpx_conv1d_wgrad. */
-int64_t hpx_conv1d_wgrad_workspace(int64_t num_samples, int cin, int cout);
+int64_t hpx_conv1d_wgrad_workspace(int64_t num_samples, int cin, int cout, int taps);
 
-/* grad_weight[co][k][ci] = sum_{s, p} dz[s][p][co] in[s][p + k - 2][ci], grad_bias[co] = sum dz (float32). */
-int hpx_conv1d_wgrad(const uint16_t* dz, const uint16_t* in, int64_t num_samples, int cin, int cout, float* workspace,
-                     float* grad_weight, float* grad_bias, void* stream);
+/* grad_weight[co][k][ci] = sum_{s, p} dz[s][p][co] in[s][p + k - taps/2][ci], grad_bias[co] = sum dz
+ * (float32). */
+int hpx_conv1d_wgrad(const uint16_t* dz, const uint16_t* in, int64_t num_samples, int cin, int cout, int taps,
+                     float* workspace, float* grad_weight, float* grad_bias, void* stream);
 
 /* dz = bf16(grad * gelu'(pre)) elementwise. */
 int hpx_gelu_backward(const float* grad, const float* pre, int64_t n, uint16_t* dz, void* stream);
@@ -302,9 +305,10 @@ int hpx_to_bf16(const float* x, int64_t n, uint16_t* y, void* stream);
 
 /* Network input from pilot LS values ls [num_tx][num_drops][num_rx][pilot_words * 64] (hpx_pilot_ls):
  * x[s][p][2 v + {0, 1}] = {re, im} ls[..][v * 64 + p], x[s][p][2 pilot_words] = -1 / +1 for stream
- * 0 / others, zero up to cin channels. */
+ * 0 / others, x[s][p][2 pilot_words + 1] = -log10(noise_power[d]) / 3 (0 if noise_power is NULL),
+ * zero up to cin channels. */
 int hpx_net_input(const hpx_c64* ls, int64_t num_drops, int num_rx, int num_tx, int pilot_words, int cin,
-                  uint16_t* x, void* stream);
+                  const double* noise_power, uint16_t* x, void* stream);
 
 /* Channel estimates est [num_samples][words * N] from the network output out [num_samples][64][2 words B]
  * (B = N / 64 subcarriers per position): est[s][w][B p + u] = out[s][p][2 (w B + u) + {0, 1}]. */
@@ -330,23 +334,24 @@ int hpx_nmse_grad(const float* out, const hpx_c64* csi, const hpx_c64* spatial, 
  * hpx_pilot_ls, num_pilots = 64 x pilot words), the four layers and the skip convolution with the
  * activations in LDS, and the estimates est [num_tx][num_drops][num_rx][words * N].  weights /
  * biases: host arrays of 5 device pointers (layers 1-4, then the skip convolution), bfloat16
- * [cout][5][cin] with cin = 8, 64, 64, 64, 8 and float32 [cout]; cout = 64 for the hidden layers and
- * 2 words N / 64 for the last two.  Same arithmetic as hpx_net_input + hpx_conv1d_forward (x4, then
- * mode 2) + hpx_net_output. */
+ * [cout][taps][cin] with (cin, taps) = (8, 5), (64, 5), (64, 5), (64, 5), (8, 17) and float32 [cout];
+ * cout = 64 for the hidden layers and 2 words N / 64 for the last two; noise_power [num_drops] feeds
+ * the SNR input channel.  Same arithmetic as hpx_net_input + hpx_conv1d_forward (x4, then mode 2)
+ * + hpx_net_output. */
 int hpx_learned_estimate(const hpx_c64* grid, int64_t num_drops, int num_rx, int num_tx, int words,
                          int num_subcarriers, const int32_t* pilot_index, const hpx_c64* pilot_values, int num_pilots,
-                         const uint16_t* const* weights, const float* const* biases, int cout, hpx_c64* est,
-                         void* stream);
+                         const double* noise_power, const uint16_t* const* weights, const float* const* biases,
+                         int cout, hpx_c64* est, void* stream);
 
 /* One Adam step (bias-corrected, step >= 1) on float32 parameters. */
 int hpx_adam_step(float* param, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1, float beta2,
                   float eps, int64_t step, void* stream);
 
-/* bfloat16 copies of float32 weights [cout][5][cin]: weight (same layout) and weight_transposed
- * [cin][5][cout] with weight_transposed[ci][k][co] = w[co][4 - k][ci] (the backward-data
+/* bfloat16 copies of float32 weights [cout][taps][cin]: weight (same layout) and weight_transposed
+ * [cin][taps][cout] with weight_transposed[ci][k][co] = w[co][taps - 1 - k][ci] (the backward-data
  * convolution's weights); either may be NULL. */
-int hpx_conv1d_weight_copies(const float* param, int cout, int cin, uint16_t* weight, uint16_t* weight_transposed,
-                             void* stream);
+int hpx_conv1d_weight_copies(const float* param, int cout, int cin, int taps, uint16_t* weight,
+                             uint16_t* weight_transposed, void* stream);
 
 /* ---------------------------------------------------------------- heterogeneous batches
  * Thousands of drops with different antenna counts, FFT sizes, path counts, Doppler,

@@ -46,26 +46,27 @@ def test_pilot_ls_and_lmmse_apply(cuda):
                 assert rel_err(out[t, dd, r], want) < 1e-5
 
 
-@pytest.mark.parametrize("cin,cout,mode", [(8, 64, 1), (64, 64, 1), (64, 224, 0), (224, 64, 0)])
-def test_conv1d_forward_matches_torch(cuda, cin, cout, mode):
+@pytest.mark.parametrize("cin,cout,mode,taps", [(8, 64, 1, 5), (64, 64, 1, 5), (64, 224, 0, 5), (224, 64, 0, 5),
+                                                (8, 224, 0, 17)])
+def test_conv1d_forward_matches_torch(cuda, cin, cout, mode, taps):
     from hermespy_amd import _lib
 
-    rng = np.random.default_rng(cin + cout)
+    rng = np.random.default_rng(cin + cout + taps)
     s = 3
     x = bf16(rng.standard_normal((s, 64, cin)))
-    w = bf16(rng.standard_normal((cout, 5, cin)) / np.sqrt(5 * cin))
+    w = bf16(rng.standard_normal((cout, taps, cin)) / np.sqrt(taps * cin))
     b = rng.standard_normal(cout).astype(np.float32)
     xd = torch.as_tensor(x, device=cuda).to(torch.bfloat16)
     wd = torch.as_tensor(w, device=cuda).to(torch.bfloat16)
     bd = torch.as_tensor(b, device=cuda)
     of = torch.empty((s, 64, cout), dtype=torch.float32, device=cuda)
     ob = torch.empty((s, 64, cout), dtype=torch.bfloat16, device=cuda)
-    _lib.check(_lib.load().hpx_conv1d_forward(_lib.ptr(xd), s, cin, _lib.ptr(wd), _lib.ptr(bd), cout, mode,
+    _lib.check(_lib.load().hpx_conv1d_forward(_lib.ptr(xd), s, cin, taps, _lib.ptr(wd), _lib.ptr(bd), cout, mode,
                                               _lib.ptr(of), _lib.ptr(ob), _lib.stream_handle()))
     torch.cuda.synchronize()
     ref = F.conv1d(torch.as_tensor(x, dtype=torch.float64).permute(0, 2, 1),
                    torch.as_tensor(w, dtype=torch.float64).permute(0, 2, 1),
-                   torch.as_tensor(b, dtype=torch.float64), padding=2).permute(0, 2, 1).numpy()
+                   torch.as_tensor(b, dtype=torch.float64), padding=taps // 2).permute(0, 2, 1).numpy()
     assert rel_err(of.cpu().numpy(), ref) < 1e-5
     act = F.gelu(torch.as_tensor(ref), approximate="tanh").numpy() if mode == 1 else ref
     got = ob.float().cpu().numpy()
@@ -91,14 +92,14 @@ def test_conv1d_backward_matches_torch(cuda):
     _lib.check(lib.hpx_gelu_backward(_lib.ptr(gd), _lib.ptr(pd), gd.numel(), _lib.ptr(dz), st))
     w16 = torch.empty((cout, 5, cin), dtype=torch.bfloat16, device=cuda)
     wt16 = torch.empty((cin, 5, cout), dtype=torch.bfloat16, device=cuda)
-    _lib.check(lib.hpx_conv1d_weight_copies(_lib.ptr(wfd), cout, cin, _lib.ptr(w16), _lib.ptr(wt16), st))
-    ws = torch.empty(int(lib.hpx_conv1d_wgrad_workspace(s, cin, cout)) // 4 + 1, dtype=torch.float32, device=cuda)
+    _lib.check(lib.hpx_conv1d_weight_copies(_lib.ptr(wfd), cout, cin, 5, _lib.ptr(w16), _lib.ptr(wt16), st))
+    ws = torch.empty(int(lib.hpx_conv1d_wgrad_workspace(s, cin, cout, 5)) // 4 + 1, dtype=torch.float32, device=cuda)
     gw = torch.empty((cout, 5, cin), dtype=torch.float32, device=cuda)
     gb = torch.empty(cout, dtype=torch.float32, device=cuda)
-    _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(xd), s, cin, cout, _lib.ptr(ws), _lib.ptr(gw), _lib.ptr(gb),
-                                    st))
+    _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(xd), s, cin, cout, 5, _lib.ptr(ws), _lib.ptr(gw),
+                                    _lib.ptr(gb), st))
     gx = torch.empty((s, 64, cin), dtype=torch.float32, device=cuda)
-    _lib.check(lib.hpx_conv1d_forward(_lib.ptr(dz), s, cout, _lib.ptr(wt16), None, cin, 0, _lib.ptr(gx), None, st))
+    _lib.check(lib.hpx_conv1d_forward(_lib.ptr(dz), s, cout, 5, _lib.ptr(wt16), None, cin, 0, _lib.ptr(gx), None, st))
     torch.cuda.synchronize()
 
     pre64 = torch.as_tensor(pre, dtype=torch.float64).requires_grad_(True)
@@ -164,7 +165,7 @@ def test_learned_estimator_trains(cuda):
 
     cfg = mimo2x2_512_pilots()
     net = ConvNetEstimator(cfg)
-    assert 115_000 < net.num_params < 135_000
+    assert 135_000 < net.num_params < 155_000
     tr = Trainer(net, drops_per_step=66, lr=3e-3)
     first = 1 << 30
     before = float(tr.step(first, lr=0.0))
@@ -174,7 +175,7 @@ def test_learned_estimator_trains(cuda):
     assert np.isfinite(before) and after < before - 1.0, (before, after)
     # the loss from the estimate path: per-drop error and power, grouped by SNR point
     ls = tr.simulate(first)
-    est = net.estimate(ls)
+    est = net.estimate(ls, tr.noise_power)
     e = torch.zeros(66, dtype=torch.float64, device=cuda)
     p = torch.zeros(66, dtype=torch.float64, device=cuda)
     _lib.check(_lib.load().hpx_estimate_error(_lib.ptr(est), _lib.ptr(tr.csi), _lib.ptr(tr.spatial), 66, 2, 2,
@@ -183,3 +184,26 @@ def test_learned_estimator_trains(cuda):
     en, pn = e.cpu().numpy(), p.cpu().numpy()
     want = np.mean([10 * np.log10(en[g == k].sum() / pn[g == k].sum()) for k in range(len(cfg.snr_db))])
     assert abs(want - after) < 1e-3, (want, after)
+
+
+def test_skip_wgrad_17_taps_matches_torch(cuda):
+    from hermespy_amd import _lib
+
+    lib, st = _lib.load(), _lib.stream_handle()
+    rng = np.random.default_rng(8)
+    s, cin, cout, taps = 40, 8, 224, 17
+    x = bf16(rng.standard_normal((s, 64, cin)))
+    dz = bf16(rng.standard_normal((s, 64, cout)))
+    xd = torch.as_tensor(x, device=cuda).to(torch.bfloat16)
+    dzd = torch.as_tensor(dz, device=cuda).to(torch.bfloat16)
+    ws = torch.empty(int(lib.hpx_conv1d_wgrad_workspace(s, cin, cout, taps)) // 4 + 1, dtype=torch.float32, device=cuda)
+    gw = torch.empty((cout, taps, cin), dtype=torch.float32, device=cuda)
+    gb = torch.empty(cout, dtype=torch.float32, device=cuda)
+    _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dzd), _lib.ptr(xd), s, cin, cout, taps, _lib.ptr(ws), _lib.ptr(gw),
+                                    _lib.ptr(gb), st))
+    torch.cuda.synchronize()
+    xt = torch.as_tensor(x, dtype=torch.float64).permute(0, 2, 1)
+    dzt = torch.as_tensor(dz, dtype=torch.float64).permute(0, 2, 1)
+    ref = torch.nn.grad.conv1d_weight(xt, (cout, cin, taps), dzt, padding=taps // 2).permute(0, 2, 1).numpy()
+    assert rel_err(gw.cpu().numpy(), ref) < 1e-5
+    assert rel_err(gb.cpu().numpy(), dz.sum(axis=(0, 1))) < 1e-5

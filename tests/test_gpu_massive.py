@@ -90,6 +90,6 @@ def test_learned_fused_estimate_matches_layer_path(cuda):
     grid = torch.empty((n, 2, 14, 512), dtype=torch.complex64, device=cuda)
     bl.run(SEED, 0, npw, err, doppler=cfg.draw_doppler(0, n), grid=grid)
     lm = LMMSEChannelEstimation(cfg, snr_db=[12.0])
-    a = net.estimate(lm.pilot_ls(grid, n)).cpu().numpy()
-    f = net.estimate_grid(grid, lm.pilot_index, lm.pilot_values).cpu().numpy()
+    a = net.estimate(lm.pilot_ls(grid, n), npw).cpu().numpy()
+    f = net.estimate_grid(grid, lm.pilot_index, lm.pilot_values, npw).cpu().numpy()
     assert rel_err(f, a) < 1e-5
