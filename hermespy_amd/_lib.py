@@ -133,9 +133,9 @@ SIGNATURES = {
     "hpx_lmmse_apply": (c_int, [c_vp, c_int, c_i64, c_int, c_vp, c_i64, c_vp, c_int, c_vp, c_vp]),
     "hpx_detect_estimated": (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_int, c_int,
                                      c_vp, c_u64, c_i64, c_vp, c_vp, c_vp]),
-    "hpx_conv1d_forward": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
+    "hpx_conv1d_forward": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp]),
     "hpx_conv1d_wgrad_workspace": (c_i64, [c_i64, c_int, c_int, c_int]),
-    "hpx_conv1d_wgrad": (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "hpx_conv1d_wgrad": (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
     "hpx_gelu_backward": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
     "hpx_to_bf16": (c_int, [c_vp, c_i64, c_vp, c_vp]),
     "hpx_net_input": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp]),

@@ -200,7 +200,8 @@ def comb_pilot_frame(num_subcarriers: int, words: int, pilot_words, spacing: int
 def mimo2x2_512_pilots() -> LinkConfig:
     """2x2 correlated (rho=0.5) 23-tap TDL over 3 us (the 4x4 link's channel) at 15.36 MHz, 512 subcarriers,
     CP 64 (N/8), 14 words with comb QPSK pilots on every 4th subcarrier of words 2 and 11, Doppler
-    uniform in 10..500 Hz per drop, 16-QAM, MMSE."""
+    uniform in 10..500 Hz per drop, 16-QAM, MMSE.  The grid has no DC roll (``dc_suppression=False``,
+    an OFDMWaveform option), so each stream's comb samples the frequency response uniformly."""
 
     fs = 15.36e6
     tau = np.arange(23) * 3e-6 / 22
@@ -212,7 +213,7 @@ def mimo2x2_512_pilots() -> LinkConfig:
     return LinkConfig(2, 2, 512, 14, 64 / 512, 16, fs, tau, p / p.sum(), k, 2 * np.pi * 500.0, corr_rx=r, corr_tx=r,
                       detector="mmse", name="mimo2x2-512sc-16qam-comb-pilots",
                       snr_db=tuple(float(s) for s in range(0, 31, 3)), frame_mask=mask, pilots=pilots,
-                      doppler_hz_range=(10.0, 500.0))
+                      doppler_hz_range=(10.0, 500.0), dc_suppression=False)
 
 
 CONFIGS = {"siso-64": siso_64, "mimo4x4-1024": mimo4x4_1024, "mimo2x2-512-pilots": mimo2x2_512_pilots}

@@ -5,8 +5,11 @@
 // convolution, weight gradients, NMSE loss, Adam).
 //
 // Layout: sample s = (t * num_drops + d) * num_rx + r, positions p = 0..63,
-// channels innermost: activations [S][64][C].  Weights [Cout][5][Cin] (kernel 5,
-// zero padding 2).  The network is the learned counterpart of
+// channels innermost: activations [S][64][C].  Weights [Cout][taps][Cin].  The
+// positions are cyclic (a stream's pilots on a grid without DC roll sample the
+// periodic frequency response uniformly), so every convolution is circular:
+// out[p] = sum_k w[k] in[(p + (k - taps/2) dil) mod 64], with dilations 1, 4, 16, 1
+// in the four layers (a receptive field covering the whole axis) and 1 in the skip.  The network is the learned counterpart of
 // OrthogonalLeastSquaresChannelEstimation's interpolation
 // (hermespy/modem/waveforms/orthogonal/channel.py:29-66): it maps the LS values at
 // a stream's pilots to the channel of every resource element.
@@ -25,10 +28,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kPos = 64;   // pilot positions per sample
-constexpr int kHalo = 2;   // kernel 5 of the four layers
-constexpr int kTaps = 5;
+constexpr int kTaps = 5;       // the four layers
 constexpr int kSkipTaps = 17;  // the linear skip convolution
-constexpr int kSkipHalo = kSkipTaps / 2;
+constexpr int kDil1 = 1, kDil2 = 4, kDil3 = 16, kDil4 = 1;
 
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
@@ -59,18 +61,16 @@ __device__ __forceinline__ float gelu_grad(float x) {
 template <int CIN, int TAPS>
 __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t* __restrict__ in,
                                                        const uint16_t* __restrict__ w,
-                                                       const float* __restrict__ bias, int cout, int mode,
+                                                       const float* __restrict__ bias, int cout, int dil, int mode,
                                                        float* __restrict__ out_f32, uint16_t* __restrict__ out_bf16) {
   constexpr int kStride = CIN + 4;  // padded LDS row: conflict-free 8-byte reads across a half wave
-  constexpr int kH = TAPS / 2;
-  __shared__ __align__(16) uint16_t xs[(kPos + 2 * kH) * kStride];
+  __shared__ __align__(16) uint16_t xs[kPos * kStride];
   const int s = blockIdx.x;
   const int tid = threadIdx.x;
   const uint16_t* xin = in + static_cast<int64_t>(s) * kPos * CIN;
-  for (int i = tid; i < (kPos + 2 * kH) * CIN; i += 256) {
+  for (int i = tid; i < kPos * CIN; i += 256) {
     const int row = i / CIN, c = i - row * CIN;
-    const int p = row - kH;
-    xs[row * kStride + c] = (p >= 0 && p < kPos) ? xin[p * CIN + c] : static_cast<uint16_t>(0);
+    xs[row * kStride + c] = xin[i];
   }
   __syncthreads();
   const int wave = tid >> 6, lane = tid & 63;
@@ -82,12 +82,12 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t* __restric
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
     const int co = ct * 32 + m;
     const uint16_t* wrow = w + static_cast<int64_t>(co) * TAPS * CIN + 4 * kq;
-    const uint16_t* arow = xs + (32 * half + m) * kStride + 4 * kq;
 #pragma unroll
     for (int k = 0; k < TAPS; ++k) {
+      const uint16_t* arow = xs + ((32 * half + m + (k - TAPS / 2) * dil) & (kPos - 1)) * kStride + 4 * kq;
 #pragma unroll 4
       for (int c0 = 0; c0 < CIN; c0 += 8) {
-        const bf16x4 a = *reinterpret_cast<const bf16x4*>(arow + k * kStride + c0);
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(arow + c0);
         const bf16x4 b = *reinterpret_cast<const bf16x4*>(wrow + k * CIN + c0);
         acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
       }
@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) to_bf16_kernel(const float* __restrict__ 
 constexpr int kWgSamples = 32;
 
 __global__ void __launch_bounds__(256) wgrad_kernel(const uint16_t* __restrict__ dz, const uint16_t* __restrict__ in,
-                                                    int num_samples, int cout, int cin, int taps,
+                                                    int num_samples, int cout, int cin, int taps, int dil,
                                                     float* __restrict__ part, float* __restrict__ part_bias) {
   __shared__ float dzs[kPos][33];
   __shared__ float xs[kPos][65];
@@ -155,8 +155,8 @@ __global__ void __launch_bounds__(256) wgrad_kernel(const uint16_t* __restrict__
       float v = 0.0f;
       if (col < kcols) {
         const int k = col / cin, ci = col - k * cin;
-        const int q = p + k - halo;  // input position feeding output p through tap k
-        if (q >= 0 && q < kPos) v = bf2f(in[(static_cast<int64_t>(s) * kPos + q) * cin + ci]);
+        const int q = (p + (k - halo) * dil) & (kPos - 1);  // input position feeding output p through tap k
+        v = bf2f(in[(static_cast<int64_t>(s) * kPos + q) * cin + ci]);
       }
       xs[p][c] = v;
     }
@@ -398,37 +398,37 @@ __global__ void __launch_bounds__(256) weight_copies_kernel(const float* __restr
 // layers use the 32x32x16 MFMA (8 channels per lane and step).
 constexpr int kFusedSamples = 2;
 constexpr int kS8 = 8 + 4, kS64 = 64 + 8;  // padded LDS rows
-constexpr int kXRows = kPos + 2 * kSkipHalo;   // input rows (halo of the 17-tap skip)
-constexpr int kARows = kPos + 2 * kHalo;       // hidden activation rows
 
-// 8-channel input (32x32x8 steps): taps k = 0..TAPS-1 read rows ROW0 + p + k
-template <int TAPS, int ROW0>
-__device__ __forceinline__ void fused_tile8(const uint16_t (*in)[kXRows * kS8], const uint16_t* __restrict__ w, int co,
-                                            int half, int m, int kq, floatx16 (&acc)[kFusedSamples]) {
+// 8-channel input (32x32x8 steps): tap k reads row (p + (k - TAPS/2) dil) mod 64
+template <int TAPS>
+__device__ __forceinline__ void fused_tile8(const uint16_t (*in)[kPos * kS8], const uint16_t* __restrict__ w, int co,
+                                            int dil, int half, int m, int kq, floatx16 (&acc)[kFusedSamples]) {
   const uint16_t* wrow = w + static_cast<int64_t>(co) * TAPS * 8 + 4 * kq;
 #pragma unroll 1
   for (int k = 0; k < TAPS; ++k) {
     const bf16x4 b = *reinterpret_cast<const bf16x4*>(wrow + k * 8);
+    const int row = (32 * half + m + (k - TAPS / 2) * dil) & (kPos - 1);
 #pragma unroll
     for (int g = 0; g < kFusedSamples; ++g) {
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(&in[g][(ROW0 + 32 * half + m + k) * kS8 + 4 * kq]);
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(&in[g][row * kS8 + 4 * kq]);
       acc[g] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[g], 0, 0, 0);
     }
   }
 }
 
 // 64-channel input (32x32x16 steps: lanes 0-31 take channels c0..c0+7, lanes 32-63 c0+8..c0+15)
-__device__ __forceinline__ void fused_tile64(const uint16_t (*in)[kARows * kS64], const uint16_t* __restrict__ w,
-                                             int co, int half, int m, int kq, floatx16 (&acc)[kFusedSamples]) {
+__device__ __forceinline__ void fused_tile64(const uint16_t (*in)[kPos * kS64], const uint16_t* __restrict__ w, int co,
+                                             int dil, int half, int m, int kq, floatx16 (&acc)[kFusedSamples]) {
   const uint16_t* wrow = w + static_cast<int64_t>(co) * kTaps * 64 + 8 * kq;
 #pragma unroll 1
   for (int k = 0; k < kTaps; ++k) {
+    const int row = (32 * half + m + (k - kTaps / 2) * dil) & (kPos - 1);
 #pragma unroll
     for (int c0 = 0; c0 < 64; c0 += 16) {
       const bf16x8 b = *reinterpret_cast<const bf16x8*>(wrow + k * 64 + c0);
 #pragma unroll
       for (int g = 0; g < kFusedSamples; ++g) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&in[g][(32 * half + m + k) * kS64 + 8 * kq + c0]);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&in[g][row * kS64 + 8 * kq + c0]);
         acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[g], 0, 0, 0);
       }
     }
@@ -443,14 +443,14 @@ __device__ __forceinline__ void zero_acc(floatx16 (&acc)[kFusedSamples]) {
 }
 
 __device__ __forceinline__ void store_hidden(const floatx16 (&acc)[kFusedSamples], const float* __restrict__ bias,
-                                             int co, int half, int kq, uint16_t (*out)[kARows * kS64]) {
+                                             int co, int half, int kq, uint16_t (*out)[kPos * kS64]) {
   const float bv = bias[co];
 #pragma unroll
   for (int g = 0; g < kFusedSamples; ++g)
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int p = 32 * half + (r >> 2) * 8 + kq * 4 + (r & 3);
-      out[g][(p + kHalo) * kS64 + co] = f2bf(gelu(acc[g][r] + bv));
+      out[g][p * kS64 + co] = f2bf(gelu(acc[g][r] + bv));
     }
 }
 
@@ -461,33 +461,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) l
     const float* __restrict__ b2, const uint16_t* __restrict__ w3, const float* __restrict__ b3,
     const uint16_t* __restrict__ w4, const float* __restrict__ b4, const uint16_t* __restrict__ wsk,
     const float* __restrict__ bsk, int cout, float* __restrict__ est) {
-  __shared__ __align__(16) uint16_t xs[kFusedSamples][kXRows * kS8];
-  __shared__ __align__(16) uint16_t a0[kFusedSamples][kARows * kS64];
-  __shared__ __align__(16) uint16_t a1[kFusedSamples][kARows * kS64];
+  __shared__ __align__(16) uint16_t xs[kFusedSamples][kPos * kS8];
+  __shared__ __align__(16) uint16_t a0[kFusedSamples][kPos * kS64];
+  __shared__ __align__(16) uint16_t a1[kFusedSamples][kPos * kS64];
   const int tid = threadIdx.x;
   const int64_t s0 = static_cast<int64_t>(blockIdx.x) * kFusedSamples;
   const int64_t wn = static_cast<int64_t>(words) * n;
   const int64_t dr_count = static_cast<int64_t>(num_drops) * num_rx;
   const int pilot_words = num_pilots / kPos;
-  // zero halo rows
-  for (int i = tid; i < kFusedSamples * 2 * kHalo * kS64; i += 256) {
-    const int g = i / (2 * kHalo * kS64), rest = i - g * 2 * kHalo * kS64;
-    const int row = rest / kS64, c = rest - row * kS64;
-    const int rr = row < kHalo ? row : kPos + row;
-    a0[g][rr * kS64 + c] = 0;
-    a1[g][rr * kS64 + c] = 0;
-  }
-  for (int i = tid; i < kFusedSamples * 2 * kSkipHalo * kS8; i += 256) {
-    const int g = i / (2 * kSkipHalo * kS8), rest = i - g * 2 * kSkipHalo * kS8;
-    const int row = rest / kS8, c = rest - row * kS8;
-    const int rr = row < kSkipHalo ? row : kPos + row;
-    xs[g][rr * kS8 + c] = 0;
-  }
   // input: LS at the sample's own pilots, two channels per pilot word, the stream flag, the SNR
   for (int i = tid; i < kFusedSamples * kPos; i += 256) {
     const int g = i / kPos, p = i - g * kPos;
     const int64_t s = s0 + g;
-    uint16_t* row = &xs[g][(p + kSkipHalo) * kS8];
+    uint16_t* row = &xs[g][p * kS8];
     if (s >= num_samples) {
       for (int c = 0; c < 8; ++c) row[c] = 0;
       continue;
@@ -514,21 +500,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) l
   {  // layer 1: 8 -> 64 (column tile wave / 2)
     const int co = (wave >> 1) * 32 + m;
     zero_acc(acc);
-    fused_tile8<kTaps, kSkipHalo - kHalo>(xs, w1, co, half, m, kq, acc);
+    fused_tile8<kTaps>(xs, w1, co, kDil1, half, m, kq, acc);
     store_hidden(acc, b1, co, half, kq, a0);
   }
   __syncthreads();
   {
     const int co = (wave >> 1) * 32 + m;
     zero_acc(acc);
-    fused_tile64(a0, w2, co, half, m, kq, acc);
+    fused_tile64(a0, w2, co, kDil2, half, m, kq, acc);
     store_hidden(acc, b2, co, half, kq, a1);
   }
   __syncthreads();
   {
     const int co = (wave >> 1) * 32 + m;
     zero_acc(acc);
-    fused_tile64(a1, w3, co, half, m, kq, acc);
+    fused_tile64(a1, w3, co, kDil3, half, m, kq, acc);
     store_hidden(acc, b3, co, half, kq, a0);
   }
   __syncthreads();
@@ -538,10 +524,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) l
   for (int ct = wave >> 1; ct < cout / 32; ct += 2) {
     const int co = ct * 32 + m;
     zero_acc(acc);
-    fused_tile64(a0, w4, co, half, m, kq, acc);
+    fused_tile64(a0, w4, co, kDil4, half, m, kq, acc);
     floatx16 sk[kFusedSamples];
     zero_acc(sk);
-    fused_tile8<kSkipTaps, 0>(xs, wsk, co, half, m, kq, sk);
+    fused_tile8<kSkipTaps>(xs, wsk, co, 1, half, m, kq, sk);
     const float bv = b4[co], bs = bsk[co];
     const int c2 = co >> 1, reim = co & 1;
     const int w = c2 / blk, u = c2 - w * blk;
@@ -562,9 +548,10 @@ unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); 
 
 }  // namespace
 
-extern "C" int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, int taps, const uint16_t* weight,
-                                  const float* bias, int cout, int mode, float* out_f32, uint16_t* out_bf16,
-                                  void* stream) {
+extern "C" int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, int taps, int dilation,
+                                  const uint16_t* weight, const float* bias, int cout, int mode, float* out_f32,
+                                  uint16_t* out_bf16, void* stream) {
+  HPX_REQUIRE(dilation >= 1 && dilation < kPos, "hpx_conv1d_forward: dilation must be in [1, 63]");
   HPX_REQUIRE(num_samples >= 0 && num_samples <= 0x7fffffff, "hpx_conv1d_forward: invalid sample count");
   HPX_REQUIRE(cout >= 32 && cout % 32 == 0, "hpx_conv1d_forward: cout must be a multiple of 32 (got %d)", cout);
   HPX_REQUIRE(mode >= 0 && mode <= 2, "hpx_conv1d_forward: mode must be 0 (linear), 1 (GELU) or 2 (accumulate)");
@@ -573,7 +560,8 @@ extern "C" int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int c
   const dim3 g(static_cast<unsigned>(num_samples));
   const hipStream_t s = as_stream(stream);
 #define HPX_CONV(CIN, TAPS) \
-  hipLaunchKernelGGL((conv_fwd_kernel<CIN, TAPS>), g, dim3(256), 0, s, in, weight, bias, cout, mode, out_f32, out_bf16)
+  hipLaunchKernelGGL((conv_fwd_kernel<CIN, TAPS>), g, dim3(256), 0, s, in, weight, bias, cout, dilation, mode, \
+                     out_f32, out_bf16)
   if (cin == 8 && taps == kTaps) HPX_CONV(8, kTaps);
   else if (cin == 8 && taps == kSkipTaps) HPX_CONV(8, kSkipTaps);
   else if (cin == 64 && taps == kTaps) HPX_CONV(64, kTaps);
@@ -593,7 +581,9 @@ extern "C" int64_t hpx_conv1d_wgrad_workspace(int64_t num_samples, int cin, int 
 }
 
 extern "C" int hpx_conv1d_wgrad(const uint16_t* dz, const uint16_t* in, int64_t num_samples, int cin, int cout,
-                                int taps, float* workspace, float* grad_weight, float* grad_bias, void* stream) {
+                                int taps, int dilation, float* workspace, float* grad_weight, float* grad_bias,
+                                void* stream) {
+  HPX_REQUIRE(dilation >= 1 && dilation < kPos, "hpx_conv1d_wgrad: dilation must be in [1, 63]");
   HPX_REQUIRE(num_samples >= 1 && num_samples <= 0x7fffffff && cin >= 1 && cout >= 1 && taps >= 1 && (taps & 1) &&
                   taps < kPos,
               "hpx_conv1d_wgrad: invalid sizes");
@@ -605,8 +595,8 @@ extern "C" int hpx_conv1d_wgrad(const uint16_t* dz, const uint16_t* in, int64_t 
   const hipStream_t s = as_stream(stream);
   const dim3 g(static_cast<unsigned>(chunks), static_cast<unsigned>((cout + 31) / 32),
                static_cast<unsigned>((taps * cin + 63) / 64));
-  hipLaunchKernelGGL(wgrad_kernel, g, dim3(256), 0, s, dz, in, static_cast<int>(num_samples), cout, cin, taps, part_w,
-                     part_b);
+  hipLaunchKernelGGL(wgrad_kernel, g, dim3(256), 0, s, dz, in, static_cast<int>(num_samples), cout, cin, taps,
+                     dilation, part_w, part_b);
   hipLaunchKernelGGL(reduce_parts_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, part_w, chunks, nw, grad_weight);
   hipLaunchKernelGGL(reduce_parts_kernel, dim3(blocks_for(cout)), dim3(256), 0, s, part_b, chunks,
                      static_cast<int64_t>(cout), grad_bias);

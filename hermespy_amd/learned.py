@@ -4,10 +4,12 @@ A 1-D convolutional network over the 64 pilot positions of one (receive,
 transmit) stream pair maps the LS values at that stream's pilots (two pilot
 words, real and imaginary parts, a stream flag and the SNR: 6 of 8 input channels) to
 the channel of every resource element: 64 positions x (words x N / 64
-subcarriers x re/im) outputs.  Four layers, kernel 5, 64 hidden channels, GELU
-(tanh form), plus a linear 17-tap skip convolution from the input to the
-output (the linear interpolator the nonlinear path refines): about 146k
-parameters.  Weights and activations are bfloat16,
+subcarriers x re/im) outputs.  Four circular layers, kernel 5, dilations 1, 4,
+16, 1 (the receptive field covers the whole cyclic pilot axis, which a linear
+estimator needs to reach LMMSE: the frequency response is periodic and, without
+the DC roll, uniformly sampled by a stream's comb pilots), 64 hidden channels,
+GELU (tanh form), plus a linear 17-tap skip convolution from the input to the
+output: about 146k parameters.  Weights and activations are bfloat16,
 products accumulate in float32 on the MFMA units (``hpx_conv1d_forward``);
 training runs on the device with hand-written kernels: backward-data as the
 forward convolution with transposed, flipped weights, float32 weight
@@ -39,8 +41,11 @@ CIN = 8  # 2 pilot words x (re, im) + stream flag + SNR, zero-padded to 8
 WEIGHTS_PATH = Path(__file__).resolve().parent / "data" / "learned_estimator.npz"
 
 
+DILATIONS = (1, 4, 16, 1)
+
+
 def layer_shapes(cfg: LinkConfig) -> list:
-    """(cin, cout, gelu) of the four layers."""
+    """(cin, cout, gelu) of the four layers (dilations DILATIONS)."""
 
     cout = 2 * cfg.words * (cfg.num_subcarriers // POSITIONS)
     return [(CIN, HIDDEN, True), (HIDDEN, HIDDEN, True), (HIDDEN, HIDDEN, True), (HIDDEN, cout, False)]
@@ -82,6 +87,8 @@ class ConvNetEstimator:
     def __init__(self, cfg: LinkConfig, params: list | None = None, device: str | torch.device = "cuda") -> None:
         if cfg.frame_mask is None or cfg.num_subcarriers % POSITIONS:
             raise ValueError("the learned estimator needs a pilot frame with a multiple of 64 subcarriers")
+        if cfg.dc_suppression:
+            raise ValueError("the learned estimator's circular convolutions need a grid without DC roll")
         self.cfg = cfg
         self.device = torch.device(device)
         self.shapes = layer_shapes(cfg)
@@ -129,15 +136,16 @@ class ConvNetEstimator:
             last = i == len(self.shapes) - 1
             pre = torch.empty((s, POSITIONS, cout), dtype=torch.float32, device=self.device) if (keep or last) else None
             nxt = None if last else torch.empty((s, POSITIONS, cout), dtype=torch.bfloat16, device=self.device)
-            _lib.check(lib.hpx_conv1d_forward(_lib.ptr(a), s, cin, TAPS, _lib.ptr(self.w16[i]), _lib.ptr(self.b[i]),
-                                              cout, 1 if act else 0, _lib.ptr(pre), _lib.ptr(nxt), st))
+            _lib.check(lib.hpx_conv1d_forward(_lib.ptr(a), s, cin, TAPS, DILATIONS[i], _lib.ptr(self.w16[i]),
+                                              _lib.ptr(self.b[i]), cout, 1 if act else 0, _lib.ptr(pre), _lib.ptr(nxt),
+                                              st))
             cache.append((a, pre))
             a = nxt
             out = pre
         # linear skip path from the input, accumulated onto the last layer's output
         cin, cout = skip_shape(self.cfg)
-        _lib.check(lib.hpx_conv1d_forward(_lib.ptr(x), s, cin, SKIP_TAPS, _lib.ptr(self.w16[-1]), _lib.ptr(self.b[-1]),
-                                          cout, 2, _lib.ptr(out), None, st))
+        _lib.check(lib.hpx_conv1d_forward(_lib.ptr(x), s, cin, SKIP_TAPS, 1, _lib.ptr(self.w16[-1]),
+                                          _lib.ptr(self.b[-1]), cout, 2, _lib.ptr(out), None, st))
         return (out, cache) if keep else out
 
     def estimate_grid(self, grid: torch.Tensor, pilot_index: torch.Tensor, pilot_values: torch.Tensor,
@@ -276,17 +284,17 @@ class Trainer:
                 _lib.check(lib.hpx_gelu_backward(_lib.ptr(grad), _lib.ptr(pre), grad.numel(), _lib.ptr(dz), st))
             else:
                 _lib.check(lib.hpx_to_bf16(_lib.ptr(grad), grad.numel(), _lib.ptr(dz), st))
-            _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(a_in), s, cin, cout, TAPS, _lib.ptr(self.workspace),
-                                            _lib.ptr(self.gw[i]), _lib.ptr(self.gb[i]), st))
+            _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(a_in), s, cin, cout, TAPS, DILATIONS[i],
+                                            _lib.ptr(self.workspace), _lib.ptr(self.gw[i]), _lib.ptr(self.gb[i]), st))
             if i == len(net.shapes) - 1:  # the skip convolution sees the same output gradient
                 scin, scout = skip_shape(c)
-                _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(x), s, scin, scout, SKIP_TAPS,
+                _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(x), s, scin, scout, SKIP_TAPS, 1,
                                                 _lib.ptr(self.workspace), _lib.ptr(self.gw[-1]), _lib.ptr(self.gb[-1]),
                                                 st))
             if i > 0:
                 grad = torch.empty((s, POSITIONS, cin), dtype=torch.float32, device=net.device)
-                _lib.check(lib.hpx_conv1d_forward(_lib.ptr(dz), s, cout, TAPS, _lib.ptr(net.wt16[i]), None, cin, 0,
-                                                  _lib.ptr(grad), None, st))
+                _lib.check(lib.hpx_conv1d_forward(_lib.ptr(dz), s, cout, TAPS, DILATIONS[i], _lib.ptr(net.wt16[i]), None,
+                                                  cin, 0, _lib.ptr(grad), None, st))
         self.step_count += 1
         b1, b2 = self.betas
         for p, g, m1, m2 in zip(net.w + net.b, self.gw + self.gb, self.m1, self.m2):

@@ -278,24 +278,27 @@ int hpx_run_batch_frames(const hpx_link* link_host, const hpx_frame* frame_host,
  * OrthogonalLeastSquaresChannelEstimation (orthogonal/channel.py:29-66); no reference
  * counterpart otherwise.  Activations are bfloat16 (uint16 bit patterns)
  * [samples][64][channels], sample = (t * num_drops + d) * num_rx + r; weights
- * bfloat16 [cout][taps][cin] (odd taps, zero padding taps / 2): 5 for the four
- * layers, 17 for the linear skip convolution from the input. */
+ * bfloat16 [cout][taps][cin] (odd taps): 5 for the four layers (dilations 1, 4, 16, 1),
+ * 17 for the linear skip convolution from the input (dilation 1).  The 64 positions
+ * are cyclic (a stream's comb pilots on a grid without DC roll sample the periodic
+ * frequency response uniformly), so every convolution is circular. */
 
-/* out[s][p][co] = bias[co] + sum_{k, ci} weight[co][k][ci] in[s][p + k - taps/2][ci] on the MFMA
- * units (float32 accumulation).  mode 0: out_f32 = that, out_bf16 = bf16(that); mode 1 (GELU, tanh
- * form): out_f32 = the pre-activation, out_bf16 = bf16(gelu(pre)); mode 2: out_f32 += that (a skip
- * path).  bias may be NULL; either output may be NULL.  (cin, taps) in {(8|64|224, 5), (8, 17)},
- * cout a multiple of 32. */
-int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, int taps, const uint16_t* weight,
-                       const float* bias, int cout, int mode, float* out_f32, uint16_t* out_bf16, void* stream);
+/* out[s][p][co] = bias[co] + sum_{k, ci} weight[co][k][ci] in[s][(p + (k - taps/2) dilation) mod 64][ci]
+ * on the MFMA units (float32 accumulation).  mode 0: out_f32 = that, out_bf16 = bf16(that); mode 1
+ * (GELU, tanh form): out_f32 = the pre-activation, out_bf16 = bf16(gelu(pre)); mode 2: out_f32 +=
+ * that (a skip path).  bias may be NULL; either output may be NULL.  (cin, taps) in
+ * {(8|64|224, 5), (8, 17)}, cout a multiple of 32. */
+int hpx_conv1d_forward(const uint16_t* in, int64_t num_samples, int cin, int taps, int dilation,
+                       const uint16_t* weight, const float* bias, int cout, int mode, float* out_f32,
+                       uint16_t* out_bf16, void* stream);
 
 /* float32 workspace bytes of hpx_conv1d_wgrad. */
 int64_t hpx_conv1d_wgrad_workspace(int64_t num_samples, int cin, int cout, int taps);
 
-/* grad_weight[co][k][ci] = sum_{s, p} dz[s][p][co] in[s][p + k - taps/2][ci], grad_bias[co] = sum dz
- * (float32). */
+/* grad_weight[co][k][ci] = sum_{s, p} dz[s][p][co] in[s][(p + (k - taps/2) dilation) mod 64][ci],
+ * grad_bias[co] = sum dz (float32). */
 int hpx_conv1d_wgrad(const uint16_t* dz, const uint16_t* in, int64_t num_samples, int cin, int cout, int taps,
-                     float* workspace, float* grad_weight, float* grad_bias, void* stream);
+                     int dilation, float* workspace, float* grad_weight, float* grad_bias, void* stream);
 
 /* dz = bf16(grad * gelu'(pre)) elementwise. */
 int hpx_gelu_backward(const float* grad, const float* pre, int64_t n, uint16_t* dz, void* stream);

@@ -46,9 +46,18 @@ def test_pilot_ls_and_lmmse_apply(cuda):
                 assert rel_err(out[t, dd, r], want) < 1e-5
 
 
-@pytest.mark.parametrize("cin,cout,mode,taps", [(8, 64, 1, 5), (64, 64, 1, 5), (64, 224, 0, 5), (224, 64, 0, 5),
-                                                (8, 224, 0, 17)])
-def test_conv1d_forward_matches_torch(cuda, cin, cout, mode, taps):
+def circular_conv(x, w, b, dil):
+    """float64 reference: x [S][64][cin], w [cout][taps][cin] -> [S][64][cout], circular, dilated."""
+
+    h = (w.shape[1] // 2) * dil
+    xt = F.pad(torch.as_tensor(x, dtype=torch.float64).permute(0, 2, 1), (h, h), mode="circular")
+    bt = None if b is None else torch.as_tensor(b, dtype=torch.float64)
+    return F.conv1d(xt, torch.as_tensor(w, dtype=torch.float64).permute(0, 2, 1), bt, dilation=dil).permute(0, 2, 1)
+
+
+@pytest.mark.parametrize("cin,cout,mode,taps,dil", [(8, 64, 1, 5, 1), (64, 64, 1, 5, 4), (64, 64, 1, 5, 16),
+                                                    (64, 224, 0, 5, 1), (224, 64, 0, 5, 16), (8, 224, 0, 17, 1)])
+def test_conv1d_forward_matches_torch(cuda, cin, cout, mode, taps, dil):
     from hermespy_amd import _lib
 
     rng = np.random.default_rng(cin + cout + taps)
@@ -61,12 +70,10 @@ def test_conv1d_forward_matches_torch(cuda, cin, cout, mode, taps):
     bd = torch.as_tensor(b, device=cuda)
     of = torch.empty((s, 64, cout), dtype=torch.float32, device=cuda)
     ob = torch.empty((s, 64, cout), dtype=torch.bfloat16, device=cuda)
-    _lib.check(_lib.load().hpx_conv1d_forward(_lib.ptr(xd), s, cin, taps, _lib.ptr(wd), _lib.ptr(bd), cout, mode,
-                                              _lib.ptr(of), _lib.ptr(ob), _lib.stream_handle()))
+    _lib.check(_lib.load().hpx_conv1d_forward(_lib.ptr(xd), s, cin, taps, dil, _lib.ptr(wd), _lib.ptr(bd), cout,
+                                              mode, _lib.ptr(of), _lib.ptr(ob), _lib.stream_handle()))
     torch.cuda.synchronize()
-    ref = F.conv1d(torch.as_tensor(x, dtype=torch.float64).permute(0, 2, 1),
-                   torch.as_tensor(w, dtype=torch.float64).permute(0, 2, 1),
-                   torch.as_tensor(b, dtype=torch.float64), padding=taps // 2).permute(0, 2, 1).numpy()
+    ref = circular_conv(x, w, b, dil).numpy()
     assert rel_err(of.cpu().numpy(), ref) < 1e-5
     act = F.gelu(torch.as_tensor(ref), approximate="tanh").numpy() if mode == 1 else ref
     got = ob.float().cpu().numpy()
@@ -75,13 +82,13 @@ def test_conv1d_forward_matches_torch(cuda, cin, cout, mode, taps):
 
 def test_conv1d_backward_matches_torch(cuda):
     """GELU layer: dz = bf16(g * gelu'(pre)); dW, db by hpx_conv1d_wgrad; dX by the forward kernel
-    on the transposed, flipped weights."""
+    on the transposed, flipped weights (circular, dilation 4)."""
 
     from hermespy_amd import _lib
 
     lib, st = _lib.load(), _lib.stream_handle()
     rng = np.random.default_rng(3)
-    s, cin, cout = 70, 64, 224
+    s, cin, cout, dil = 70, 64, 224, 4
     x = bf16(rng.standard_normal((s, 64, cin)))
     wf = (rng.standard_normal((cout, 5, cin)) / np.sqrt(5 * cin)).astype(np.float32)
     pre = rng.standard_normal((s, 64, cout)).astype(np.float32)
@@ -96,10 +103,11 @@ def test_conv1d_backward_matches_torch(cuda):
     ws = torch.empty(int(lib.hpx_conv1d_wgrad_workspace(s, cin, cout, 5)) // 4 + 1, dtype=torch.float32, device=cuda)
     gw = torch.empty((cout, 5, cin), dtype=torch.float32, device=cuda)
     gb = torch.empty(cout, dtype=torch.float32, device=cuda)
-    _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(xd), s, cin, cout, 5, _lib.ptr(ws), _lib.ptr(gw),
+    _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dz), _lib.ptr(xd), s, cin, cout, 5, dil, _lib.ptr(ws), _lib.ptr(gw),
                                     _lib.ptr(gb), st))
     gx = torch.empty((s, 64, cin), dtype=torch.float32, device=cuda)
-    _lib.check(lib.hpx_conv1d_forward(_lib.ptr(dz), s, cout, 5, _lib.ptr(wt16), None, cin, 0, _lib.ptr(gx), None, st))
+    _lib.check(lib.hpx_conv1d_forward(_lib.ptr(dz), s, cout, 5, dil, _lib.ptr(wt16), None, cin, 0, _lib.ptr(gx), None,
+                                      st))
     torch.cuda.synchronize()
 
     pre64 = torch.as_tensor(pre, dtype=torch.float64).requires_grad_(True)
@@ -107,14 +115,15 @@ def test_conv1d_backward_matches_torch(cuda):
     dz_ref = bf16(pre64.grad.numpy())
     got_dz = dz.float().cpu().numpy()
     assert np.all(np.abs(got_dz - dz_ref) <= 2.0 ** -7 * np.abs(dz_ref) + 1e-6)
-    dzt = torch.as_tensor(got_dz, dtype=torch.float64).permute(0, 2, 1)
-    xt = torch.as_tensor(x, dtype=torch.float64).permute(0, 2, 1)
-    w_bf = torch.as_tensor(bf16(wf), dtype=torch.float64).permute(0, 2, 1)
-    gw_ref = torch.nn.grad.conv1d_weight(xt, w_bf.shape, dzt, padding=2).permute(0, 2, 1).numpy()
-    gx_ref = torch.nn.grad.conv1d_input(xt.shape, w_bf, dzt, padding=2).permute(0, 2, 1).numpy()
-    assert rel_err(gw.cpu().numpy(), gw_ref) < 1e-5
+    # autograd of the circular dilated convolution on the same bf16 operands
+    xt = torch.as_tensor(x, dtype=torch.float64).requires_grad_(True)
+    wt = torch.as_tensor(bf16(wf), dtype=torch.float64).requires_grad_(True)
+    h = 2 * dil
+    out = F.conv1d(F.pad(xt.permute(0, 2, 1), (h, h), mode="circular"), wt.permute(0, 2, 1), dilation=dil)
+    out.backward(torch.as_tensor(got_dz, dtype=torch.float64).permute(0, 2, 1))
+    assert rel_err(gw.cpu().numpy(), wt.grad.numpy()) < 1e-5
     assert rel_err(gb.cpu().numpy(), got_dz.sum(axis=(0, 1))) < 1e-5
-    assert rel_err(gx.cpu().numpy(), gx_ref) < 1e-5
+    assert rel_err(gx.cpu().numpy(), xt.grad.numpy()) < 1e-5
     assert np.array_equal(wt16.float().cpu().numpy(), np.flip(bf16(wf), axis=1).transpose(2, 1, 0))
 
 
@@ -199,11 +208,14 @@ def test_skip_wgrad_17_taps_matches_torch(cuda):
     ws = torch.empty(int(lib.hpx_conv1d_wgrad_workspace(s, cin, cout, taps)) // 4 + 1, dtype=torch.float32, device=cuda)
     gw = torch.empty((cout, taps, cin), dtype=torch.float32, device=cuda)
     gb = torch.empty(cout, dtype=torch.float32, device=cuda)
-    _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dzd), _lib.ptr(xd), s, cin, cout, taps, _lib.ptr(ws), _lib.ptr(gw),
+    _lib.check(lib.hpx_conv1d_wgrad(_lib.ptr(dzd), _lib.ptr(xd), s, cin, cout, taps, 1, _lib.ptr(ws), _lib.ptr(gw),
                                     _lib.ptr(gb), st))
     torch.cuda.synchronize()
-    xt = torch.as_tensor(x, dtype=torch.float64).permute(0, 2, 1)
-    dzt = torch.as_tensor(dz, dtype=torch.float64).permute(0, 2, 1)
-    ref = torch.nn.grad.conv1d_weight(xt, (cout, cin, taps), dzt, padding=taps // 2).permute(0, 2, 1).numpy()
+    xt = torch.as_tensor(x, dtype=torch.float64)
+    wt = torch.zeros((cout, taps, cin), dtype=torch.float64, requires_grad=True)
+    circular_conv_t = F.conv1d(F.pad(xt.permute(0, 2, 1), (taps // 2, taps // 2), mode="circular"),
+                               wt.permute(0, 2, 1))
+    circular_conv_t.backward(torch.as_tensor(dz, dtype=torch.float64).permute(0, 2, 1))
+    ref = wt.grad.numpy()
     assert rel_err(gw.cpu().numpy(), ref) < 1e-5
     assert rel_err(gb.cpu().numpy(), dz.sum(axis=(0, 1))) < 1e-5
