@@ -219,3 +219,42 @@ def test_skip_wgrad_17_taps_matches_torch(cuda):
     ref = wt.grad.numpy()
     assert rel_err(gw.cpu().numpy(), ref) < 1e-5
     assert rel_err(gb.cpu().numpy(), dz.sum(axis=(0, 1))) < 1e-5
+
+
+def test_trained_estimator_tracks_lmmse(cuda):
+    """The committed weights: on 440 fresh drops (40 per SNR point) the learned estimator's NMSE is
+    within 0.5 dB of the LMMSE fallback's at every SNR point (it is trained to beat it)."""
+
+    from hermespy_amd import _lib
+    from hermespy_amd.configs import mimo2x2_512_pilots
+    from hermespy_amd.learned import ConvNetEstimator
+    from hermespy_amd.lmmse import LMMSEChannelEstimation
+    from hermespy_amd.simulation import BatchedLink
+
+    cfg = mimo2x2_512_pilots()
+    n = 440
+    snr = np.asarray(cfg.snr_db)
+    sidx = np.arange(n) % snr.size
+    npw = torch.as_tensor(cfg.noise_power(snr[sidx]), dtype=torch.float64, device=cuda)
+    bl = BatchedLink(cfg)
+    err = torch.empty(n, dtype=torch.int32, device=cuda)
+    grid = torch.empty((n, 2, 14, 512), dtype=torch.complex64, device=cuda)
+    csi = torch.empty((n, 14, 512), dtype=torch.complex64, device=cuda)
+    sp = torch.empty((n, 2, 2), dtype=torch.complex64, device=cuda)
+    first = 3_000_000  # away from the training drops (from 2^30) and the bench drops (from 0)
+    bl.run(SEED, first, npw, err, doppler=cfg.draw_doppler(first, n), grid=grid, csi=csi, spatial=sp)
+    lm = LMMSEChannelEstimation(cfg)
+    net = ConvNetEstimator.load(cfg)
+    lib, st = _lib.load(), _lib.stream_handle()
+
+    def nmse(est):
+        e = torch.zeros(n, dtype=torch.float64, device=cuda)
+        p = torch.zeros(n, dtype=torch.float64, device=cuda)
+        _lib.check(lib.hpx_estimate_error(_lib.ptr(est), _lib.ptr(csi), _lib.ptr(sp), n, 2, 2, 14 * 512, _lib.ptr(e),
+                                          _lib.ptr(p), st))
+        en, pn = e.cpu().numpy(), p.cpu().numpy()
+        return np.array([10 * np.log10(en[sidx == k].sum() / pn[sidx == k].sum()) for k in range(snr.size)])
+
+    a = nmse(lm.apply(lm.pilot_ls(grid, n), sidx))
+    b = nmse(net.estimate_grid(grid, lm.pilot_index, lm.pilot_values, npw))
+    assert np.all(b - a < 0.5), (a.round(2), b.round(2))
