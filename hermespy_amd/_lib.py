@@ -16,7 +16,7 @@ import torch  # noqa: F401  (must precede the library load, see module docstring
 
 LIB_PATH = Path(__file__).resolve().parent / "_native" / "libhermespy_amd.so"
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 HPX_OK = 0
 HPX_ERR_INVALID = -1
@@ -69,6 +69,18 @@ class HpxBatchOut(ctypes.Structure):
     """Mirror of ``hpx_batch_out`` (device pointers)."""
 
     _fields_ = [("errors", c_vp), ("equalized", c_vp), ("grid", c_vp), ("csi", c_vp), ("spatial", c_vp)]
+
+
+class HpxDropInputs(ctypes.Structure):
+    """Mirror of ``hpx_drop_inputs`` (device pointers)."""
+
+    _fields_ = [("noise_power", c_vp), ("doppler", c_vp), ("drop_index", c_vp), ("normals", c_vp)]
+
+
+class HpxSectionStats(ctypes.Structure):
+    """Mirror of ``hpx_section_stats`` (device pointers)."""
+
+    _fields_ = [("sum", c_vp), ("sum_sq", c_vp), ("count", c_vp), ("errors", c_vp), ("bits", c_vp)]
 
 
 class HpxDropTables(ctypes.Structure):
@@ -129,6 +141,12 @@ SIGNATURES = {
     "hpx_spatial_detector_host": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
     "hpx_stream_bits": (c_int, [c_u64, c_u32, c_u32, c_i64, c_vp, c_vp]),
     "hpx_realization_normals": (c_int, [c_u64, c_u32, c_i64, c_vp, c_vp]),
+    "hpx_realization_normals_batch": (c_int, [c_u64, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]),
+    "hpx_run_batch_drops": (c_int, [ctypes.POINTER(HpxLink), ctypes.POINTER(HpxFrame), c_vp, c_vp, c_vp, c_vp,
+                                    ctypes.POINTER(HpxDropInputs), c_u64, c_i64, c_i64, ctypes.POINTER(HpxBatchOut),
+                                    c_vp]),
+    "hpx_section_accumulate": (c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, c_i64, c_i32,
+                                       ctypes.POINTER(HpxSectionStats), c_vp]),
     "hpx_pilot_ls": (c_int, [c_vp, c_i64, c_int, c_int, c_i64, c_int, c_vp, c_vp, c_vp, c_vp]),
     "hpx_lmmse_apply": (c_int, [c_vp, c_int, c_i64, c_int, c_vp, c_i64, c_vp, c_int, c_vp, c_vp]),
     "hpx_detect_estimated": (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_int, c_int,

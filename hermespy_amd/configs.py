@@ -111,6 +111,13 @@ class LinkConfig:
         return int(self.delays.size)
 
     @property
+    def num_normals(self) -> int:
+        """Standard normals of one realisation: the 10 x 10 spatial block, then the LOS angles,
+        NLOS angles, LOS phases and NLOS phases (``consistent.py:410-432``, ``fading.py:468-515``)."""
+
+        return 100 + 2 * self.num_paths + 2 * self.num_paths * self.num_sinusoids
+
+    @property
     def delay_samples(self) -> np.ndarray:
         return np.rint(self.delays * self.bandwidth).astype(np.int32)  # fading.py:297
 

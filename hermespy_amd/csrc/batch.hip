@@ -57,6 +57,8 @@ struct BatchArgs {
   const double* corr_tx;
   const double* noise_power;
   const double* doppler_drop;  // [drop][2] NLOS, LOS Doppler, or null (the link's)
+  const uint32_t* drop_index;  // [drop] Philox drop counters, or null (first_drop + drop)
+  const double* normals;       // [drop][nnorm] recorded realisation normals, or null (drawn)
   int32_t* errors;
   float2* equalized;
   // frame (null elements: every element is data)
@@ -82,7 +84,8 @@ __device__ __forceinline__ uint32_t symbol_bits(const uint32_t* bw, int bit, int
 }
 
 // One drop, simulated by the whole workgroup.  local_drop indexes the per-drop arrays
-// (noise power, Doppler, errors, exports) and offsets the Philox drop index first_drop.
+// (noise power, Doppler, errors, exports); the Philox drop index is drop_index[local_drop]
+// or first_drop + local_drop.
 template <int NT, int NR, int PAIRS>
 __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t local_drop) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -105,16 +108,22 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
 
   const int tid = threadIdx.x;
   const int nthreads = blockDim.x;
-  const uint32_t drop = static_cast<uint32_t>(a.first_drop + local_drop);
+  const uint32_t drop = a.drop_index ? a.drop_index[local_drop] : static_cast<uint32_t>(a.first_drop + local_drop);
   const int n = a.n;
   const int L = a.paths;
 
   // ---------------------------------------------------------------- prologue
-  // realisation uniforms Phi(z), z ~ N(0,1) (consistent.py:432,485)
-  for (int c = tid; 2 * c < a.nnorm; c += nthreads) {
-    const double2 z = normal_pair_f64(stream_call(a.seed, drop, HPX_STREAM_REALIZATION, static_cast<uint32_t>(c)));
-    uni[2 * c] = normal_cdf(z.x);
-    if (2 * c + 1 < a.nnorm) uni[2 * c + 1] = normal_cdf(z.y);
+  // realisation uniforms Phi(z), z ~ N(0,1) (consistent.py:432,485), or z of a recorded
+  // StaticConsistentRealization (consistent.py:244-260)
+  if (a.normals) {
+    const double* z = a.normals + local_drop * a.nnorm;
+    for (int c = tid; c < a.nnorm; c += nthreads) uni[c] = normal_cdf(z[c]);
+  } else {
+    for (int c = tid; 2 * c < a.nnorm; c += nthreads) {
+      const double2 z = normal_pair_f64(stream_call(a.seed, drop, HPX_STREAM_REALIZATION, static_cast<uint32_t>(c)));
+      uni[2 * c] = normal_cdf(z.x);
+      if (2 * c + 1 < a.nnorm) uni[2 * c + 1] = normal_cdf(z.y);
+    }
   }
   if (tid == 0) red[0] = 0;
   __syncthreads();
@@ -804,12 +813,25 @@ extern "C" int hpx_run_batch_frames(const hpx_link* link_host, const hpx_frame* 
                                     const double* path_gains, const hpx_c128* corr_rx, const hpx_c128* corr_tx,
                                     const double* doppler, uint64_t seed, int64_t first_drop, int64_t num_drops,
                                     const double* noise_power, const hpx_batch_out* out_host, void* stream) {
-  HPX_REQUIRE(out_host, "hpx_run_batch: null output descriptor");
+  hpx_drop_inputs in{};
+  in.noise_power = noise_power;
+  in.doppler = doppler;
+  return hpx_run_batch_drops(link_host, frame_host, delays, path_gains, corr_rx, corr_tx, &in, seed, first_drop,
+                             num_drops, out_host, stream);
+}
+
+extern "C" int hpx_run_batch_drops(const hpx_link* link_host, const hpx_frame* frame_host, const int32_t* delays,
+                                   const double* path_gains, const hpx_c128* corr_rx, const hpx_c128* corr_tx,
+                                   const hpx_drop_inputs* in_host, uint64_t seed, int64_t first_drop,
+                                   int64_t num_drops, const hpx_batch_out* out_host, void* stream) {
+  HPX_REQUIRE(out_host && in_host, "hpx_run_batch: null input or output descriptor");
   int32_t* errors = out_host->errors;
+  const double* noise_power = in_host->noise_power;
+  const double* doppler = in_host->doppler;
   HPX_REQUIRE(delays && path_gains && noise_power && errors, "hpx_run_batch: null pointer");
   HPX_REQUIRE((corr_rx == nullptr) == (corr_tx == nullptr), "hpx_run_batch: give both correlation matrices or none");
   HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff, "hpx_run_batch: invalid drop count");
-  HPX_REQUIRE(first_drop >= 0 && first_drop + num_drops <= (int64_t(1) << 32),
+  HPX_REQUIRE(in_host->drop_index || (first_drop >= 0 && first_drop + num_drops <= (int64_t(1) << 32)),
               "hpx_run_batch: drop index exceeds 32 bits");
   Plan plan;
   const int st = make_plan(link_host, frame_host, &plan);
@@ -824,6 +846,8 @@ extern "C" int hpx_run_batch_frames(const hpx_link* link_host, const hpx_frame* 
   a.errors = errors;
   a.equalized = reinterpret_cast<float2*>(out_host->equalized);
   a.doppler_drop = doppler;
+  a.drop_index = in_host->drop_index;
+  a.normals = in_host->normals;
   a.elements = frame_host ? frame_host->elements : nullptr;
   a.word_data = frame_host ? frame_host->word_data : nullptr;
   a.pilots = frame_host ? reinterpret_cast<const float2*>(frame_host->pilots) : nullptr;

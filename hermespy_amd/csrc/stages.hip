@@ -598,6 +598,35 @@ extern "C" int hpx_realization_normals(uint64_t seed, uint32_t drop, int64_t cou
   return check_launch("hpx_realization_normals");
 }
 
+// grid (calls, drops): normals of several drops, each row as hpx_realization_normals writes it
+__global__ void __launch_bounds__(256) realization_normals_batch_kernel(uint64_t seed, const uint32_t* __restrict__ ids,
+                                                                        int64_t first, int64_t count,
+                                                                        double* __restrict__ out) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t d = blockIdx.y;
+  if (2 * c >= count) return;
+  const uint32_t drop = ids ? ids[d] : static_cast<uint32_t>(first + d);
+  const double2 z = normal_pair_f64(stream_call(seed, drop, HPX_STREAM_REALIZATION, static_cast<uint32_t>(c)));
+  double* row = out + d * count;
+  row[2 * c] = z.x;
+  if (2 * c + 1 < count) row[2 * c + 1] = z.y;
+}
+
+extern "C" int hpx_realization_normals_batch(uint64_t seed, const uint32_t* drop_index, int64_t first_drop,
+                                             int64_t num_drops, int64_t count, double* normals, void* stream) {
+  HPX_REQUIRE(count >= 0 && count <= (int64_t(1) << 24), "hpx_realization_normals_batch: invalid count");
+  HPX_REQUIRE(num_drops >= 0 && num_drops <= 65535, "hpx_realization_normals_batch: at most 65535 drops per call");
+  HPX_REQUIRE(drop_index || (first_drop >= 0 && first_drop + num_drops <= (int64_t(1) << 32)),
+              "hpx_realization_normals_batch: drop index exceeds 32 bits");
+  if (count == 0 || num_drops == 0) return HPX_OK;
+  HPX_REQUIRE(normals, "hpx_realization_normals_batch: null pointer");
+  const int64_t calls = (count + 1) / 2;
+  hipLaunchKernelGGL(realization_normals_batch_kernel,
+                     dim3(static_cast<unsigned>((calls + 255) / 256), static_cast<unsigned>(num_drops)), dim3(256), 0,
+                     HPX_STREAM(stream), seed, drop_index, first_drop, count, normals);
+  return check_launch("hpx_realization_normals_batch");
+}
+
 // ------------------------------------------------------------------ pilot LS estimation
 __global__ void __launch_bounds__(256) ls_pilots_kernel(const float2* __restrict__ grid, int64_t ld, int npil,
                                                         const int32_t* __restrict__ pidx,

@@ -291,7 +291,65 @@ class MMSEChannelEqualization:
 
 
 class BitErrorEvaluator:
-    """``evaluators.py:236-270``: bit errors between transmitted and received bits."""
+    """``evaluators.py:236-270``: bit errors between transmitted and received bits.
+
+    confidence, tolerance and min_num_samples configure the early stopping of the campaign
+    result (``CommunicationEvaluator.__init__``, ``evaluators.py:54-85``, and
+    ``ScalarEvaluator``, ``hermespy/core/pymonte/scalar.py:263-372``); the modems are accepted
+    for signature compatibility (the fused drop path evaluates its own link)."""
+
+    abbreviation = "BER"
+    title = "Bit Error Rate Evaluation"
+
+    def __init__(self, transmitting_modem=None, receiving_modem=None, confidence: float = 1.0,
+                 tolerance: float = 0.0, min_num_samples: int = 1024, plot_scale: str = "log",
+                 tick_format: str = "lin", plot_surface: bool = True) -> None:
+        self.transmitting_modem = transmitting_modem
+        self.receiving_modem = receiving_modem
+        self.confidence = confidence
+        self.tolerance = tolerance
+        self.min_num_samples = min_num_samples
+        self.plot_scale = plot_scale
+        self.tick_format = tick_format
+        self.plot_surface = plot_surface
+
+    @property
+    def confidence(self) -> float:
+        return self._confidence
+
+    @confidence.setter
+    def confidence(self, value: float) -> None:
+        if value < 0.0 or value > 1.0:
+            raise ValueError("Confidence level must be in the interval between zero and one")
+        self._confidence = float(value)
+
+    @property
+    def tolerance(self) -> float:
+        return self._tolerance
+
+    @tolerance.setter
+    def tolerance(self, value: float) -> None:
+        if value < 0.0:
+            raise ValueError("Tolerance must be greater or equal to zero")
+        self._tolerance = float(value)
+
+    @property
+    def min_num_samples(self) -> int:
+        return self._min_num_samples
+
+    @min_num_samples.setter
+    def min_num_samples(self, value: int) -> None:
+        if value < 2:
+            raise ValueError("Minimum number of samples must be at least two")
+        self._min_num_samples = int(value)
+
+    def initialize_result(self, grid, device="cuda"):
+        """``ScalarEvaluator.initialize_result`` (``scalar.py:286-290``) on device counters."""
+
+        from .campaign import ScalarEvaluationResult
+
+        return ScalarEvaluationResult(grid, self, self.tolerance, self.confidence, self.min_num_samples,
+                                      self.plot_surface, device=device)
 
     def evaluate(self, transmitted: torch.Tensor, received: torch.Tensor, out: torch.Tensor | None = None):
         n = max(transmitted.numel(), received.numel())

@@ -84,8 +84,13 @@ class BatchedLink:
     def run(self, seed: int, first_drop: int, noise_power: torch.Tensor, errors: torch.Tensor,
             equalized: torch.Tensor | None = None, stream: int | None = None, doppler=None,
             grid: torch.Tensor | None = None, csi: torch.Tensor | None = None,
-            spatial: torch.Tensor | None = None) -> None:
+            spatial: torch.Tensor | None = None, drop_index: torch.Tensor | None = None,
+            normals: torch.Tensor | None = None) -> None:
         """Launch ``noise_power.numel()`` drops (asynchronous on the current stream).
+
+        drop_index: int32 tensor holding the uint32 Philox drop counter of each drop (bit
+        pattern), instead of first_drop + i.  normals: float64 (num_drops, num_normals)
+        realisation normals (recorded ``scalar_samples``) replacing the Philox draw.
 
         doppler: per-drop (NLOS, LOS) doppler_frequency, (num_drops, 2) float64 (host array or device
         tensor), bounded by the link's Doppler.  grid / csi / spatial: optional complex64 exports of the
@@ -112,12 +117,19 @@ class BatchedLink:
             elif doppler.dtype != torch.float64 or doppler.numel() != 2 * n:
                 raise ValueError("per-drop Doppler must be a float64 (num_drops, 2) tensor")
             self._doppler = doppler  # held until the next launch
+        if drop_index is not None and (drop_index.dtype != torch.int32 or drop_index.numel() < n
+                                       or not drop_index.is_contiguous()):
+            raise ValueError("drop_index must be a contiguous int32 tensor (uint32 bit patterns) per drop")
+        if normals is not None and (normals.dtype != torch.float64 or not normals.is_contiguous()
+                                    or normals.numel() < n * lk.num_normals):
+            raise ValueError(f"normals must be a contiguous float64 tensor of {lk.num_normals} per drop")
         lib = _lib.load()
         out = _lib.HpxBatchOut(_lib.ptr(errors), _lib.ptr(equalized), _lib.ptr(grid), _lib.ptr(csi), _lib.ptr(spatial))
-        st = lib.hpx_run_batch_frames(
+        inputs = _lib.HpxDropInputs(_lib.ptr(noise_power), _lib.ptr(doppler), _lib.ptr(drop_index), _lib.ptr(normals))
+        st = lib.hpx_run_batch_drops(
             ctypes.byref(self.c_link), ctypes.byref(self.c_frame) if self.c_frame is not None else None,
             _lib.ptr(self.delays), _lib.ptr(self.path_gains), _lib.ptr(self.corr_rx), _lib.ptr(self.corr_tx),
-            _lib.ptr(doppler), ctypes.c_uint64(seed & (2**64 - 1)), int(first_drop), n, _lib.ptr(noise_power),
+            ctypes.byref(inputs), ctypes.c_uint64(seed & (2**64 - 1)), int(first_drop), n,
             ctypes.byref(out), _lib.stream_handle() if stream is None else stream)
         _lib.check(st)
 
@@ -139,6 +151,18 @@ class BatchedSimulation:
         snr = np.asarray(snr_db, dtype=np.float64)
         idx = (first_drop + np.arange(count)) % snr.size
         return torch.as_tensor(self.link.noise_power(snr[idx]), dtype=torch.float64, device=self.device)
+
+    def simulate(self, num_samples: int, evaluator=None, snr_db=None, max_drops_per_launch: int = 16384,
+                 record=None):
+        """``Simulation.run()`` with the reference's confidence-based stopping per SNR section
+        (``hermespy/core/pymonte/scalar.py:100-125``) on device counters; returns a
+        :class:`hermespy_amd.campaign.MonteCarloResult`.  Sample k of SNR section s is drop
+        k * len(snr_db) + s, the drop :meth:`run` simulates at that index."""
+
+        from .campaign import run_campaign
+
+        snr = self.link.snr_db if snr_db is None else snr_db
+        return run_campaign(self.batched, self.seed, snr, num_samples, evaluator, max_drops_per_launch, record)
 
     def run(self, num_drops: int, snr_db=None, drops_per_launch: int = 4096, first_drop: int = 0) -> BatchResult:
         snr = np.asarray(self.link.snr_db if snr_db is None else snr_db, dtype=np.float64)

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define HPX_ABI_VERSION 3
+#define HPX_ABI_VERSION 4
 
 typedef enum hpx_status {
   HPX_OK = 0,
@@ -271,6 +271,56 @@ int hpx_run_batch_frames(const hpx_link* link_host, const hpx_frame* frame_host,
                          const double* path_gains, const hpx_c128* corr_rx, const hpx_c128* corr_tx,
                          const double* doppler, uint64_t seed, int64_t first_drop, int64_t num_drops,
                          const double* noise_power, const hpx_batch_out* out_host, void* stream);
+
+/* Per-drop inputs of hpx_run_batch_drops (device arrays). */
+typedef struct hpx_drop_inputs {
+  const double* noise_power;  /* float64 [num_drops] (required) */
+  const double* doppler;      /* float64 [num_drops][2] {NLOS, LOS}, or NULL (as hpx_run_batch_frames) */
+  const uint32_t* drop_index; /* [num_drops] Philox drop counters of bits, noise and realisation,
+                                 or NULL for first_drop + i */
+  const double* normals;      /* float64 [num_drops][100 + 2 L + 2 L num_sinusoids]: the realisation's
+                                 standard normals z, i.e. a recorded StaticConsistentRealization's
+                                 scalar_samples (hermespy/channel/consistent.py:244-260) in the
+                                 generator's variable order (fading.py:468-515); NULL draws them */
+} hpx_drop_inputs;
+
+/* hpx_run_batch_frames with drops named by an index array and, optionally, channel
+ * realisations replayed from recorded normals.  The drop loop over grid sections of
+ * MonteCarloActor.__run_batch (hermespy/core/pymonte/actors.py:316-452) with the
+ * section order of MonteCarloQueueManager.next_batch (actors.py:77-108), and the
+ * realisation replay of Scenario.replay / drop (hermespy/core/scenario.py:441-506,817-833). */
+int hpx_run_batch_drops(const hpx_link* link_host, const hpx_frame* frame_host, const int32_t* delays,
+                        const double* path_gains, const hpx_c128* corr_rx, const hpx_c128* corr_tx,
+                        const hpx_drop_inputs* in_host, uint64_t seed, int64_t first_drop, int64_t num_drops,
+                        const hpx_batch_out* out_host, void* stream);
+
+/* Realisation normals of many drops: normals[i][c] = hpx_realization_normals(seed,
+ * drop_index ? drop_index[i] : first_drop + i)[c], c < count (device arrays). */
+int hpx_realization_normals_batch(uint64_t seed, const uint32_t* drop_index, int64_t first_drop, int64_t num_drops,
+                                  int64_t count, double* normals, void* stream);
+
+/* ---------------------------------------------------------------- Monte Carlo sections
+ * Per-section statistics of the per-drop bit error rate e_i / b_i, the artifact that
+ * BitErrorEvaluation.artifact hands to ScalarEvaluationResult.add_artifact
+ * (hermespy/modem/evaluators.py:231-233, hermespy/core/pymonte/scalar.py:100-125):
+ * the running sum, squared sum and count behind the mean and the Bayer et al.
+ * stopping rule, plus the pooled error and bit counts. */
+typedef struct hpx_section_stats {
+  double* sum;     /* [num_sections] sum of e_i / b_i */
+  double* sum_sq;  /* [num_sections] sum of (e_i / b_i)^2 */
+  int64_t* count;  /* [num_sections] drops */
+  int64_t* errors; /* [num_sections] sum of e_i */
+  int64_t* bits;   /* [num_sections] sum of b_i */
+} hpx_section_stats;
+
+/* Adds num_drops drops to the statistics (device arrays, accumulated in place; clear them
+ * to start).  Drop i belongs to section[i], or to (first_drop + i) % num_sections when
+ * section is NULL; drops whose section lies outside [0, num_sections) are skipped.  b_i is
+ * bits_per_drop[i], or bits when bits_per_drop is NULL.  One workgroup per section sums its
+ * drops in a fixed order, so results do not depend on timing. */
+int hpx_section_accumulate(const int32_t* errors, const int32_t* section, int64_t first_drop,
+                           const int32_t* bits_per_drop, int64_t bits, int64_t num_drops, int32_t num_sections,
+                           const hpx_section_stats* stats_host, void* stream);
 
 /* ---------------------------------------------------------------- learned estimator
  * A 1-D convolutional network over the 64 pilot positions of each (transmit,

@@ -1,0 +1,119 @@
+"""Monte Carlo campaigns on the device: section statistics, drops named by index, channel
+realisations replayed from recorded normals, and the early-stopping campaign against the
+sequential oracle fed with the same per-drop error counts."""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import montecarlo, philox
+
+pytestmark = pytest.mark.gpu
+
+
+def _stats(num_sections, dev):
+    from hermespy_amd import _lib
+
+    t = [torch.zeros(num_sections, dtype=torch.float64, device=dev) for _ in range(2)]
+    t += [torch.zeros(num_sections, dtype=torch.int64, device=dev) for _ in range(3)]
+    return t, _lib.HpxSectionStats(*[_lib.ptr(x) for x in t])
+
+
+@pytest.mark.parametrize("mode", ["implicit", "explicit", "per_drop_bits"])
+def test_section_accumulate_matches_numpy(cuda, mode):
+    from hermespy_amd import _lib
+
+    print(f"section accumulate {mode}", flush=True)
+    rng = np.random.default_rng(11)
+    n, S, first = 5003, 11, 7
+    err = rng.integers(0, 229376, n).astype(np.int32)
+    bits = rng.integers(1000, 300000, n).astype(np.int32) if mode == "per_drop_bits" else np.full(n, 229376, np.int32)
+    sec = (first + np.arange(n)) % S if mode == "implicit" else rng.integers(-2, S + 2, n)
+    t, c = _stats(S, cuda)
+    e_t = torch.as_tensor(err, device=cuda)
+    s_t = None if mode == "implicit" else torch.as_tensor(sec.astype(np.int32), device=cuda)
+    b_t = torch.as_tensor(bits, device=cuda) if mode == "per_drop_bits" else None
+    for half in (slice(0, 2500), slice(2500, n)):  # two launches accumulate
+        f = first + half.start
+        _lib.check(_lib.load().hpx_section_accumulate(
+            _lib.ptr(e_t[half]), _lib.ptr(None if s_t is None else s_t[half]), f,
+            _lib.ptr(None if b_t is None else b_t[half]), 229376, half.stop - half.start, S, ctypes.byref(c), None))
+    torch.cuda.synchronize()
+    ber = err.astype(np.float64) / bits
+    for s in range(S):
+        m = sec == s
+        assert int(t[2][s]) == m.sum() and int(t[3][s]) == err[m].astype(np.int64).sum()
+        assert int(t[4][s]) == bits[m].astype(np.int64).sum()
+        np.testing.assert_allclose(float(t[0][s]), ber[m].sum(), rtol=1e-13)
+        np.testing.assert_allclose(float(t[1][s]), (ber[m] ** 2).sum(), rtol=1e-13)
+
+
+def test_indexed_drops_and_replayed_normals_match(cuda):
+    from hermespy_amd import _lib
+    from hermespy_amd.configs import mimo4x4_1024
+    from hermespy_amd.simulation import BatchedLink
+
+    print("indexed drops + normals replay (mimo4x4-1024)", flush=True)
+    cfg = mimo4x4_1024()
+    bl = BatchedLink(cfg)
+    n, seed = 96, 5
+    npw = torch.as_tensor(cfg.noise_power(np.arange(n) % 11 * 3.0), dtype=torch.float64, device=cuda)
+    err = torch.empty(n, dtype=torch.int32, device=cuda)
+    eq = torch.empty(n * cfg.num_tx * cfg.num_data, dtype=torch.complex64, device=cuda)
+    bl.run(seed, 0, npw, err, eq)
+    # the same drops in a shuffled order, named by index
+    perm = np.random.default_rng(1).permutation(n)
+    ids = torch.as_tensor(perm.astype(np.uint32).view(np.int32), device=cuda)
+    err2 = torch.empty_like(err)
+    bl.run(seed, 12345, npw[torch.as_tensor(perm, device=cuda)].contiguous(), err2, drop_index=ids)
+    assert torch.equal(err2.cpu(), err.cpu()[perm])
+    # realisation normals written by the batch entry point = the per-drop entry point = the oracle
+    normals = torch.empty((n, cfg.num_normals), dtype=torch.float64, device=cuda)
+    _lib.check(_lib.load().hpx_realization_normals_batch(seed, None, 0, n, cfg.num_normals, _lib.ptr(normals), None))
+    one = torch.empty(cfg.num_normals, dtype=torch.float64, device=cuda)
+    for d in (0, 17, n - 1):
+        _lib.check(_lib.load().hpx_realization_normals(seed, d, cfg.num_normals, _lib.ptr(one), None))
+        assert torch.equal(one, normals[d])
+        np.testing.assert_allclose(normals[d].cpu().numpy(), philox.realization_normals(seed, d, cfg.num_normals),
+                                   rtol=0, atol=1e-12)
+    # replaying those normals reproduces every drop bit for bit
+    err3 = torch.empty_like(err)
+    eq3 = torch.empty_like(eq)
+    bl.run(seed, 0, npw, err3, eq3, normals=normals)
+    assert torch.equal(err3, err) and torch.equal(eq3, eq)
+    # a perturbed realisation changes the channel
+    normals[:, :100] += 0.5
+    bl.run(seed, 0, npw, err3, eq3, normals=normals)
+    assert not torch.equal(eq3, eq)
+
+
+def test_campaign_stops_like_the_sequential_oracle(cuda):
+    from hermespy_amd.configs import mimo4x4_1024
+    from hermespy_amd.modem import BitErrorEvaluator
+    from hermespy_amd.simulation import BatchedSimulation
+
+    print("campaign with early stopping (mimo4x4-1024)", flush=True)
+    cfg = mimo4x4_1024()
+    sim = BatchedSimulation(cfg, seed=9)
+    snr = np.arange(0.0, 31.0, 3.0)
+    S, K = snr.size, 96
+    # the campaign's drops as one contiguous batch: drop k * S + s is sample k of section s
+    errors = torch.empty(S * K, dtype=torch.int32, device=cuda)
+    sim.batched.run(9, 0, sim.noise_powers(0, S * K, snr), errors)
+    e = errors.cpu().numpy().astype(np.int64)
+    bits = cfg.bits_per_drop
+    # a tolerance that lets roughly half of the sections stop after 48 drops
+    sd = np.array([np.std(e[np.arange(48) * S + s] / bits, ddof=1) for s in range(S)])
+    tol = float(np.median(1.96 * sd / np.sqrt(48)))
+    ev = BitErrorEvaluator(confidence=0.05, tolerance=tol, min_num_samples=16)
+    res = sim.simulate(K, ev, snr, max_drops_per_launch=400)
+    r = res.evaluation_results[0]
+    counts, means, _, _ = montecarlo.run_sections(lambda s, k: e[k * S + s] / bits, S, K, tol, 0.05, 16)
+    assert np.array_equal(r.num_drops(), counts)
+    np.testing.assert_allclose(r.to_array(), means, rtol=1e-12, atol=0)
+    assert counts.min() < K, counts
+    pooled = r.pooled_ber()
+    for s in range(S):
+        np.testing.assert_allclose(pooled[s], e[np.arange(counts[s]) * S + s].sum() / (counts[s] * bits), rtol=1e-15)
