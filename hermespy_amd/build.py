@@ -68,7 +68,33 @@ def build(verbose: bool = False) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, lib)
+    build_h5(verbose)
     return lib
+
+
+HDF5_ROOT = Path(os.environ.get("HPX_HDF5_ROOT", "/opt/conda"))
+H5_LIB_NAME = "libhermespy_amd_h5.so"
+
+
+def build_h5(verbose: bool = False) -> Path | None:
+    """The drop record/replay store (host C++ over libhdf5); skipped when no HDF5 is installed."""
+
+    if not (HDF5_ROOT / "include" / "hdf5.h").exists():
+        if verbose:
+            print(f"no HDF5 under {HDF5_ROOT}: {H5_LIB_NAME} not built", flush=True)
+        return None
+    out = OUT_DIR / H5_LIB_NAME
+    tmp = OUT_DIR / (H5_LIB_NAME + ".tmp")
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{INCLUDE}",
+           f"-I{HDF5_ROOT / 'include'}", str(CSRC / "record.cpp"), "-o", str(tmp), f"-L{HDF5_ROOT / 'lib'}",
+           "-lhdf5", f"-Wl,-rpath,{HDF5_ROOT / 'lib'}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed on record.cpp:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass
+from enum import IntEnum
 
 import numpy as np
 import torch
@@ -146,36 +147,172 @@ class BatchedSimulation:
         self.seed = int(seed)
         self.batched = BatchedLink(link, device)
         self.device = self.batched.device
+        self._mode = ScenarioMode.DEFAULT
+        self._recorder = None
+        self._replay = None
+        self._replay_file = None
+        self._drop_counter = 0
+        self.replay_mismatches = 0
 
+    # ---------------------------------------------------------------- record / replay
+    @property
+    def mode(self) -> "ScenarioMode":
+        return self._mode
+
+    def record(self, file, campaign: str | None = None, overwrite: bool = False, normals: bool = False) -> None:
+        """Record every following drop (``Scenario.record``, ``hermespy/core/scenario.py:368-439``):
+        drop index, noise power, errors, and with ``normals`` the realisation normals."""
+
+        from .record import DropRecorder
+
+        if self._mode != ScenarioMode.DEFAULT:
+            raise RuntimeError("Initialize a recording is only possible in default mode. "
+                               "Please stop before starting a new recording.")
+        self._recorder = DropRecorder(file, self.link, self.seed, campaign, overwrite, normals)
+        self._mode = ScenarioMode.RECORD
+        self._drop_counter = 0
+
+    def replay(self, file=None, campaign: str | None = None, drop_offset: int = 0) -> int:
+        """Replay recorded drops on the following :meth:`run` calls (``Scenario.replay``,
+        ``scenario.py:441-506``); returns the number of drops to replay."""
+
+        from .record import DropReplay
+
+        if file is None:
+            if self._replay_file is None:
+                raise ValueError("Initial replay requires a valid file path argument")
+            file = self._replay_file
+        rp = DropReplay(file, campaign, drop_offset)
+        self.stop()
+        self._replay, self._replay_file = rp, file
+        self._drop_counter = 0
+        self._mode = ScenarioMode.REPLAY
+        return rp.num_drops
+
+    @classmethod
+    def Replay(cls, file, campaign: str | None = None, drop_offset: int = 0, device: str = "cuda") -> tuple:
+        """(simulation in replay mode, number of recorded drops) from a recording alone
+        (``Scenario.Replay``, ``scenario.py:508-548``)."""
+
+        from .record import DropReplay
+
+        rp = DropReplay(file, campaign)
+        link, seed = rp.link, rp.seed
+        rp.close()
+        sim = cls(link, seed, device)
+        return sim, sim.replay(file, campaign, drop_offset)
+
+    def stop(self) -> None:
+        """End a recording or replay (``scenario.py:550-566``)."""
+
+        if self._recorder is not None:
+            self._recorder.close()
+            self._recorder = None
+        if self._replay is not None:
+            self._replay.close()
+            self._replay = None
+        self._drop_counter = 0
+        self._mode = ScenarioMode.DEFAULT
+
+    def _record_launch(self, drop_index, noise_power, errors, section=None) -> None:
+        rec = self._recorder
+        normals = None
+        if rec.normals:
+            n = int(np.asarray(drop_index).size)
+            normals = torch.empty((n, self.link.num_normals), dtype=torch.float64, device=self.device)
+            ids = torch.as_tensor(np.asarray(drop_index, dtype=np.uint32).view(np.int32), device=self.device)
+            _lib.check(_lib.load().hpx_realization_normals_batch(
+                ctypes.c_uint64(self.seed & (2**64 - 1)), _lib.ptr(ids), 0, n, self.link.num_normals,
+                _lib.ptr(normals), _lib.stream_handle()))
+        rec.record(drop_index, noise_power, errors, section=section, normals=normals)
+
+    # ---------------------------------------------------------------- campaigns
     def noise_powers(self, first_drop: int, count: int, snr_db) -> torch.Tensor:
         snr = np.asarray(snr_db, dtype=np.float64)
         idx = (first_drop + np.arange(count)) % snr.size
         return torch.as_tensor(self.link.noise_power(snr[idx]), dtype=torch.float64, device=self.device)
 
-    def simulate(self, num_samples: int, evaluator=None, snr_db=None, max_drops_per_launch: int = 16384,
-                 record=None):
+    def simulate(self, num_samples: int, evaluator=None, snr_db=None, max_drops_per_launch: int = 16384):
         """``Simulation.run()`` with the reference's confidence-based stopping per SNR section
         (``hermespy/core/pymonte/scalar.py:100-125``) on device counters; returns a
         :class:`hermespy_amd.campaign.MonteCarloResult`.  Sample k of SNR section s is drop
-        k * len(snr_db) + s, the drop :meth:`run` simulates at that index."""
+        k * len(snr_db) + s, the drop :meth:`run` simulates at that index.  In record mode every
+        launch is recorded with its sections."""
 
         from .campaign import run_campaign
 
-        snr = self.link.snr_db if snr_db is None else snr_db
-        return run_campaign(self.batched, self.seed, snr, num_samples, evaluator, max_drops_per_launch, record)
-
-    def run(self, num_drops: int, snr_db=None, drops_per_launch: int = 4096, first_drop: int = 0) -> BatchResult:
+        if self._mode == ScenarioMode.REPLAY:
+            raise RuntimeError("campaigns with early stopping generate drops; use run() to replay")
         snr = np.asarray(self.link.snr_db if snr_db is None else snr_db, dtype=np.float64)
+        hook = None
+        if self._mode == ScenarioMode.RECORD:
+            noise = self.link.noise_power(snr)
+
+            def hook(launch, errors):
+                self._record_launch(launch.drop_index, noise[launch.sections], errors, launch.sections)
+
+        return run_campaign(self.batched, self.seed, snr, num_samples, evaluator, max_drops_per_launch, hook)
+
+    def run(self, num_drops: int | None = None, snr_db=None, drops_per_launch: int = 4096,
+            first_drop: int = 0) -> BatchResult:
+        """num_drops drops, drop i at SNR point i % len(snr_db).  In record mode the drops are
+        recorded; in replay mode the next num_drops recorded drops (all remaining ones if None)
+        are simulated again from their recorded inputs, and ``replay_mismatches`` counts drops
+        whose error count differs from the recorded one."""
+
+        snr = np.asarray(self.link.snr_db if snr_db is None else snr_db, dtype=np.float64)
+        if self._mode == ScenarioMode.REPLAY:
+            return self._run_replay(num_drops, snr, drops_per_launch)
+        if num_drops is None:
+            raise ValueError("give the number of drops")
         errors = torch.empty(num_drops, dtype=torch.int32, device=self.device)
         for start in range(0, num_drops, drops_per_launch):
             cnt = min(drops_per_launch, num_drops - start)
             npw = self.noise_powers(first_drop + start, cnt, snr)
             self.batched.run(self.seed, first_drop + start, npw, errors[start:start + cnt])
+            if self._mode == ScenarioMode.RECORD:
+                ids = np.arange(first_drop + start, first_drop + start + cnt, dtype=np.uint32)
+                self._record_launch(ids, npw, errors[start:start + cnt],
+                                    ((first_drop + start + np.arange(cnt)) % snr.size).astype(np.int32))
         err = errors.cpu().numpy().astype(np.int64)
         idx = (first_drop + np.arange(num_drops)) % snr.size
         per_err = np.bincount(idx, weights=err, minlength=snr.size).astype(np.int64)
         per_drops = np.bincount(idx, minlength=snr.size).astype(np.int64)
         return BatchResult(snr, per_err, per_drops, self.link.bits_per_drop)
+
+    def _run_replay(self, num_drops, snr, drops_per_launch) -> BatchResult:
+        rp = self._replay
+        left = rp.num_drops - self._drop_counter
+        num = left if num_drops is None else int(num_drops)
+        if num > left:
+            raise ValueError(f"only {left} recorded drops are left to replay")
+        per_err = np.zeros(snr.size, dtype=np.int64)
+        per_drops = np.zeros(snr.size, dtype=np.int64)
+        errors = torch.empty(min(num, drops_per_launch), dtype=torch.int32, device=self.device)
+        dev = self.device
+        for start in range(0, num, drops_per_launch):
+            cnt = min(drops_per_launch, num - start)
+            cols = rp.read(self._drop_counter, cnt)
+            ids = torch.as_tensor(cols["drop_index"].view(np.int32), device=dev)
+            npw = torch.as_tensor(cols["noise_power"], device=dev)
+            dop = torch.as_tensor(np.ascontiguousarray(cols["doppler"]), device=dev) if "doppler" in cols else None
+            z = torch.as_tensor(np.ascontiguousarray(cols["normals"]), device=dev) if "normals" in cols else None
+            self.batched.run(self.seed, 0, npw, errors[:cnt], doppler=dop, drop_index=ids, normals=z)
+            err = errors[:cnt].cpu().numpy()
+            self.replay_mismatches += int(np.count_nonzero(err != cols["errors"]))
+            sec = cols["section"] if "section" in cols else cols["drop_index"].astype(np.int64) % snr.size
+            per_err += np.bincount(sec, weights=err.astype(np.int64), minlength=snr.size).astype(np.int64)
+            per_drops += np.bincount(sec, minlength=snr.size).astype(np.int64)
+            self._drop_counter += cnt
+        return BatchResult(snr, per_err, per_drops, self.link.bits_per_drop)
+
+
+class ScenarioMode(IntEnum):
+    """``hermespy/core/scenario.py:49-70``."""
+
+    DEFAULT = 0
+    RECORD = 1
+    REPLAY = 2
 
 
 class SingleDropLink:
