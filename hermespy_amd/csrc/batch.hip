@@ -41,6 +41,14 @@ namespace {
 
 constexpr int kBatchMaxThreads = 512;
 constexpr int kTailPerThread = 4;  // registers holding the previous word's tail
+// Complex values per thread in one FFT pass: 12 where the kernel runs in 128 registers
+// (512 threads, at most 4 antennas), which keeps the FFT's registers from spilling (5 rows of
+// 1024 points still form one group; measured 3 % faster per 4x4 1024-subcarrier drop than
+// 16); 16 where registers are ample.
+template <int NT, int MAXT>
+constexpr int fft_values() {
+  return MAXT >= 512 && NT <= 4 ? 12 : 16;
+}
 
 struct BatchArgs {
   // link
@@ -86,7 +94,7 @@ __device__ __forceinline__ uint32_t symbol_bits(const uint32_t* bw, int bit, int
 // One drop, simulated by the whole workgroup.  local_drop indexes the per-drop arrays
 // (noise power, Doppler, errors, exports); the Philox drop index is drop_index[local_drop]
 // or first_drop + local_drop.
-template <int NT, int NR, int PAIRS>
+template <int NT, int NR, int PAIRS, int FFTV>
 __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t local_drop) {
   extern __shared__ __align__(16) unsigned char lds[];
   float2* xbuf = reinterpret_cast<float2*>(lds + a.off_x);
@@ -267,7 +275,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     }
     for (int q = tid; q < n; q += nthreads) csirow[q] = make_float2(0.f, 0.f);
     __syncthreads();
-    lds_fft<+1>(xbuf + x0, NT, a.xs, a.log2n, tw);
+    lds_fft<+1, FFTV>(xbuf + x0, NT, a.xs, a.log2n, tw);
     // cyclic prefix; word means of the taps at their delays (paths sharing a delay add up,
     // taps beyond min(num_taps, N) are cut as fft(n=N) cuts them)
     // (only the last cpl = min(CP, max delay) prefix samples are ever read by the delayed paths)
@@ -392,7 +400,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     }
     __syncthreads();
     // receive FFT of every antenna and, in the same passes, the CSI row's FFT over the taps
-    lds_fft<-1>(xbuf + x0, kRows + 1, a.xs, a.log2n, tw);
+    lds_fft<-1, FFTV>(xbuf + x0, kRows + 1, a.xs, a.log2n, tw);
 
     // ideal CSI, detection, demapping and bit errors per subcarrier
     for (int q = tid; q < n; q += nthreads) {
@@ -467,7 +475,7 @@ constexpr int batch_waves_per_eu() {
 template <int NT, int NR, int PAIRS, int MAXT>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu<NT, MAXT>())))
 batch_kernel(BatchArgs a) {
-  drop_body<NT, NR, PAIRS>(a, blockIdx.x);
+  drop_body<NT, NR, PAIRS, fft_values<NT, MAXT>()>(a, blockIdx.x);
 }
 
 // Heterogeneous batches: workgroup i simulates the drop plans[i] describes; its first_drop is
@@ -475,7 +483,7 @@ batch_kernel(BatchArgs a) {
 template <int NT, int NR, int PAIRS, int MAXT>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu<NT, MAXT>())))
 batch_kernel_multi(const BatchArgs* __restrict__ plans) {
-  drop_body<NT, NR, PAIRS>(plans[blockIdx.x], 0);
+  drop_body<NT, NR, PAIRS, fft_values<NT, MAXT>()>(plans[blockIdx.x], 0);
 }
 
 int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }

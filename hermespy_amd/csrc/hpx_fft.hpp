@@ -10,7 +10,9 @@
 
 namespace hpx {
 
-// Upper bound of complex values per thread and pass (count * N <= kFftMaxPerThread * blockDim).
+// Default upper bound of complex values per thread and pass (count * N <= MAXV * blockDim).
+// It sets the registers a pass holds across its barrier (2 MAXV); callers with little
+// register headroom pass a smaller bound and transform their vectors in more groups.
 constexpr int kFftMaxPerThread = 16;
 
 template <int SIGN>
@@ -25,11 +27,11 @@ __device__ __forceinline__ float2 twiddle(const float2* tw, int idx) {
   return SIGN < 0 ? w : make_float2(w.x, -w.y);
 }
 
-template <int SIGN, int R>
+template <int SIGN, int R, int MAXV>
 __device__ __forceinline__ void fft_pass(float2* data, int count, int stride, int log2n, int log2ns,
                                          const float2* tw) {
   constexpr int kLog2R = R == 4 ? 2 : 1;
-  constexpr int kItems = kFftMaxPerThread / R;
+  constexpr int kItems = MAXV / R;
   const int log2nr = log2n - kLog2R;
   const int nr = 1 << log2nr;
   const int ns = 1 << log2ns;
@@ -82,19 +84,20 @@ __device__ __forceinline__ void fft_pass(float2* data, int count, int stride, in
 
 // data: vector v at data + v * stride; tw: N entries (LDS or global).
 // Must be called by every thread of the workgroup; ends with a barrier.
-// Vectors are transformed in groups of at most kFftMaxPerThread * blockDim.x / N.
-template <int SIGN>
+// Vectors are transformed in groups of at most MAXV * blockDim.x / N.
+template <int SIGN, int MAXV = kFftMaxPerThread>
 __device__ __forceinline__ void lds_fft(float2* data, int count, int stride, int log2n, const float2* tw) {
-  const int group = max(1, (kFftMaxPerThread * static_cast<int>(blockDim.x)) >> log2n);
+  static_assert(MAXV % 4 == 0, "MAXV must be a multiple of 4");
+  const int group = max(1, (MAXV * static_cast<int>(blockDim.x)) >> log2n);
   for (int g0 = 0; g0 < count; g0 += group) {
     const int cnt = min(group, count - g0);
     float2* base = data + g0 * stride;
     int log2ns = 0;
     while (log2n - log2ns >= 2) {
-      fft_pass<SIGN, 4>(base, cnt, stride, log2n, log2ns, tw);
+      fft_pass<SIGN, 4, MAXV>(base, cnt, stride, log2n, log2ns, tw);
       log2ns += 2;
     }
-    if (log2n - log2ns == 1) fft_pass<SIGN, 2>(base, cnt, stride, log2n, log2ns, tw);
+    if (log2n - log2ns == 1) fft_pass<SIGN, 2, MAXV>(base, cnt, stride, log2n, log2ns, tw);
   }
 }
 
