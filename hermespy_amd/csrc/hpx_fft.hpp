@@ -101,6 +101,32 @@ __device__ __forceinline__ void lds_fft(float2* data, int count, int stride, int
   }
 }
 
+// tw[k] = exp(-2 pi j k / N), k < N, float32 from float64, called by every thread of the
+// workgroup (no barrier).  Computes the N / 8 + 1 angles of the first octant and fills the
+// other seven by the symmetries cos(pi/2 - x) = sin(x) and exp(-j (x + q pi/2)) = (-j)^q exp(-j x),
+// which cuts the float64 sine/cosine evaluations eightfold (N >= 8).
+__device__ __forceinline__ void fill_twiddles_octants(float2* tw, int n) {
+  const int quarter = n >> 2, eighth = n >> 3;
+  for (int r = threadIdx.x; r <= eighth; r += blockDim.x) {
+    double sd, cd;
+    sincospi(2.0 * static_cast<double>(r) / static_cast<double>(n), &sd, &cd);
+    const float c = static_cast<float>(cd), sn = static_cast<float>(sd);
+    // angles r and quarter - r of the first quadrant: exp(-j x) = (c, -s), exp(-j (pi/2 - x)) = (s, -c)
+    const int idx[2] = {r, quarter - r};
+    const float cc[2] = {c, sn}, ss[2] = {sn, c};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && (r == 0 || idx[1] == idx[0])) continue;  // r = 0: quarter - r lies in the next quadrant
+      const int k = idx[h];
+      const float x = cc[h], y = ss[h];
+      tw[k] = make_float2(x, -y);                  // q = 0
+      tw[k + quarter] = make_float2(-y, -x);       // q = 1: times -j
+      tw[k + 2 * quarter] = make_float2(-x, y);    // q = 2: times -1
+      tw[k + 3 * quarter] = make_float2(y, x);     // q = 3: times +j
+    }
+  }
+}
+
 // Grid column q of an N-subcarrier OFDM symbol -> FFT bin (composition of the
 // DC roll and (i)fftshift of hermespy/modem/waveforms/orthogonal/ofdm.py:116-128,144-147).
 __host__ __device__ __forceinline__ int subcarrier_bin(int q, int n, int dc_suppression) {

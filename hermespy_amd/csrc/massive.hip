@@ -6,8 +6,8 @@
 //                                           SoS table), Taylor coefficients of the taps, per-word tap
 //                                           means, and the per-drop detector V, A = V^H S^H, lambda of
 //                                           S^H S (fading.py:468-515, channel_estimation.py:215-295)
-//   K1 transmit  (workgroup per drop, user) bits -> 256-QAM -> OFDM + CP -> the user's tapped delay line
-//                                           over every word's data window (modem.py:797-888,
+//   K1 transmit  (workgroup per drop, user, word) bits -> 256-QAM -> OFDM + CP -> the user's tapped
+//                                           delay line over the word's data window (modem.py:797-888,
 //                                           ofdm.py:97-130, fading.py:371-406)
 //   K2 receive   (workgroup per drop, word, antenna group)
 //                                           y_r = sum_t S[r][t] z_t + AWGN, FFT, active bins; group 0
@@ -47,7 +47,7 @@ constexpr int kRxGroup = 8;      // antennas per receive workgroup
 constexpr int kRxRows = 2;       // antennas transformed together
 constexpr int kDetElements = 64;   // resource elements per detection workgroup (32 per wave)
 constexpr int kDetThreads = 2 * kDetElements;
-constexpr int kTail = 4;         // registers per thread holding the previous word's tail
+constexpr int kXcds = 8;         // MI355X dispatches consecutive workgroups round robin over 8 XCDs
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -60,6 +60,7 @@ struct MassiveArgs {
   int paths, sinusoids, terms, nnorm, blk_cols;
   int log2b, nb, maxd, dx, cpl, x0, xs, keep_below, bits_words;
   int64_t frame_len;
+  int64_t num_pairs;  // (drop, word) pairs of the receive launch
   double bandwidth, doppler, los_doppler, tx_scale;
   const int32_t* delays;
   const double* path_gains;
@@ -184,23 +185,86 @@ __global__ void __launch_bounds__(256) massive_prologue_kernel(MassiveArgs a) {
     const double2 m = tap_mean(sos + 3 * p * a.terms, a.terms, static_cast<double>(w) * (a.n + a.cp) + a.cp, a.n);
     a.abar[ld * L * a.words + i] = make_float2(static_cast<float>(m.x), static_cast<float>(m.y));
   }
+  // The per-drop detector of spatial_detector (hpx_eig.hpp) spread over the workgroup: the
+  // Gram matrix S^H S by nt * nt threads, its Jacobi eigen-decomposition by one thread,
+  // A = V^H S^H by all threads (same sums in the same order).
+  double* work = det + nt + 2 * nt * nt + 2 * nt * nr;
+  double* gre = work;
+  double* gim = work + kEigMax * kEigMax;
+  double* vre = work + 2 * kEigMax * kEigMax;
+  double* vim = work + 3 * kEigMax * kEigMax;
+  for (int e = tid; e < nt * nt; e += nth) {
+    const int i = e / nt, j = e - i * nt;
+    double sr = 0.0, si = 0.0;
+    for (int r = 0; r < nr; ++r) {  // conj(S_ri) S_rj
+      const double ar = s64[2 * (r * nt + i)], ai = s64[2 * (r * nt + i) + 1];
+      const double br = s64[2 * (r * nt + j)], bi = s64[2 * (r * nt + j) + 1];
+      sr += ar * br + ai * bi;
+      si += ar * bi - ai * br;
+    }
+    gre[e] = sr;
+    gim[e] = si;
+  }
+  __syncthreads();
   if (tid == 0) {
-    double* lam = det;
-    double* v = lam + nt;
-    double* am = v + 2 * nt * nt;
-    spatial_detector(nr, nt, s64, lam, v, am, am + 2 * nt * nr);
-    for (int i = 0; i < nt * nt; ++i)
-      a.det_v[ld * nt * nt + i] = make_float2(static_cast<float>(v[2 * i]), static_cast<float>(v[2 * i + 1]));
-    for (int i = 0; i < nt * nr; ++i)
-      a.det_a[ld * nt * nr + i] = make_float2(static_cast<float>(am[2 * i]), static_cast<float>(am[2 * i + 1]));
+    hermitian_jacobi(nt, gre, gim, vre, vim);
     double lmax = 0.0;
-    for (int i = 0; i < nt; ++i) lmax = fmax(lmax, lam[i]);
+    for (int i = 0; i < nt; ++i) lmax = fmax(lmax, gre[i * nt + i]);
+    // numpy.linalg.pinv's cut: singular values below max(M, N) eps sigma_max are dropped
     const double rt = static_cast<double>(nr > nt ? nr : nt) * 2.220446049250313e-16;
-    for (int i = 0; i < nt; ++i) a.det_l[ld * nt + i] = lam[i] > rt * rt * lmax ? static_cast<float>(lam[i]) : 0.0f;
+    for (int i = 0; i < nt; ++i) {
+      const double lam = gre[i * nt + i];
+      a.det_l[ld * nt + i] = lam > rt * rt * lmax ? static_cast<float>(lam) : 0.0f;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nt * nt; i += nth)
+    a.det_v[ld * nt * nt + i] = make_float2(static_cast<float>(vre[i]), static_cast<float>(vim[i]));
+  for (int e = tid; e < nt * nr; e += nth) {
+    const int i = e / nr, r = e - i * nr;
+    double ar = 0.0, ai = 0.0;
+    for (int k = 0; k < nt; ++k) {  // A[i][r] = sum_k conj(V[k][i]) conj(S[r][k])
+      const double vr = vre[k * nt + i], vi = -vim[k * nt + i];
+      const double sr = s64[2 * (r * nt + k)], si = -s64[2 * (r * nt + k) + 1];
+      ar += vr * sr - vi * si;
+      ai += vr * si + vi * sr;
+    }
+    a.det_a[ld * nt * nr + e] = make_float2(static_cast<float>(ar), static_cast<float>(ai));
   }
 }
 
 // ---------------------------------------------------------------- K1: transmitter + delay line
+// One workgroup per (drop, user, word).  The delay line of word w reaches dx samples back into
+// word w - 1, so the workgroup first rebuilds word w - 1's symbol (bits, mapping, IFFT) for that
+// tail, then word w's: twice the symbol work for 14 times the workgroups of a per-user loop.
+
+// bits, mapping and IFFT of word w of user t into x[x0 .. x0 + n) (ends with a barrier)
+__device__ void massive_tx_symbol(const MassiveArgs& a, uint32_t drop, int t, int w, float2* x, const float2* tw,
+                                  uint32_t* bits, const float2* cons) {
+  const int tid = threadIdx.x, nth = blockDim.x, n = a.n;
+  const int64_t bit0 = static_cast<int64_t>(w) * a.active * a.bps;
+  const int bo = static_cast<int>(bit0 & 31);
+  const int fw = static_cast<int>(bit0 >> 5);
+  const int lw = static_cast<int>((bit0 + static_cast<int64_t>(a.active) * a.bps - 1) >> 5);
+  for (int c = (fw >> 2) + tid; c <= (lw >> 2); c += nth) {
+    const uint4 r = stream_call(a.seed, drop, HPX_STREAM_BITS + t, static_cast<uint32_t>(c));
+    const uint32_t comp[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int wi = 4 * c + k - fw;
+      if (wi >= 0 && wi <= lw - fw) bits[wi] = comp[k];
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < n; q += nth) {
+    const int el = q - a.active_lo;
+    x[a.x0 + subcarrier_bin(q, n, a.dc)] =
+        (el >= 0 && el < a.active) ? cons[symbol_bits_at(bits, bo + el * a.bps, a.bps)] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft<+1>(x + a.x0, 1, a.xs, a.log2n, tw);
+}
+
 __global__ void __launch_bounds__(kTxThreads) massive_tx_kernel(MassiveArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   float2* x = reinterpret_cast<float2*>(lds + a.k1_x);
@@ -209,8 +273,11 @@ __global__ void __launch_bounds__(kTxThreads) massive_tx_kernel(MassiveArgs a) {
   uint32_t* bits = reinterpret_cast<uint32_t*>(lds + a.k1_bits);
   float2* cons = reinterpret_cast<float2*>(lds + a.k1_cons);
   const int tid = threadIdx.x, nth = blockDim.x;
-  const int64_t ld = blockIdx.x / a.nt;
-  const int t = static_cast<int>(blockIdx.x - ld * a.nt);
+  const int64_t blk = blockIdx.x;
+  const int w = static_cast<int>(blk % a.words);
+  const int64_t lt = blk / a.words;
+  const int64_t ld = lt / a.nt;
+  const int t = static_cast<int>(lt - ld * a.nt);
   const uint32_t drop = static_cast<uint32_t>(a.first_drop + ld);
   const int n = a.n, L = a.paths, x0 = a.x0;
   const int ncoef = L * a.nb * kTaylorCoeffs;
@@ -219,70 +286,35 @@ __global__ void __launch_bounds__(kTxThreads) massive_tx_kernel(MassiveArgs a) {
     const double2 p = constellation_point(a.order, 1, bits_to_label(static_cast<uint32_t>(v), a.bps));
     cons[v] = make_float2(static_cast<float>(p.x * a.tx_scale), static_cast<float>(p.y * a.tx_scale));
   }
-  for (int k = tid; k < n; k += nth) {
-    double sn, cs;
-    sincospi(-2.0 * static_cast<double>(k) / static_cast<double>(n), &sn, &cs);
-    tw[k] = make_float2(static_cast<float>(cs), static_cast<float>(sn));
-  }
-  for (int i = tid; i < a.dx; i += nth) x[i] = make_float2(0.f, 0.f);  // silence before the frame
+  fill_twiddles_octants(tw, n);
   __syncthreads();
+  // the last dx data samples of word w - 1 (silence before the frame)
+  if (w > 0) {
+    massive_tx_symbol(a, drop, t, w - 1, x, tw, bits, cons);
+    for (int i = tid; i < a.dx; i += nth) x[i] = x[x0 + n - a.dx + i];
+  } else {
+    for (int i = tid; i < a.dx; i += nth) x[i] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  massive_tx_symbol(a, drop, t, w, x, tw, bits, cons);
+  for (int i = tid; i < a.cpl; i += nth) x[x0 - a.cpl + i] = x[x0 + n - a.cpl + i];
+  __syncthreads();
+  // tapped delay line over the data window (the receiver keeps nothing else)
   const TaylorBlocks tb{a.log2b};
-  const int period = n + a.cp;
-  float2* zt = a.z + (ld * a.nt + t) * static_cast<int64_t>(a.words) * n;
-  for (int w = 0; w < a.words; ++w) {
-    // the word's data bits (stream HPX_STREAM_BITS + t, LSB-first words)
-    const int64_t bit0 = static_cast<int64_t>(w) * a.active * a.bps;
-    const int bo = static_cast<int>(bit0 & 31);
-    const int fw = static_cast<int>(bit0 >> 5);
-    const int lw = static_cast<int>((bit0 + static_cast<int64_t>(a.active) * a.bps - 1) >> 5);
-    for (int c = (fw >> 2) + tid; c <= (lw >> 2); c += nth) {
-      const uint4 r = stream_call(a.seed, drop, HPX_STREAM_BITS + t, static_cast<uint32_t>(c));
-      const uint32_t comp[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int wi = 4 * c + k - fw;
-        if (wi >= 0 && wi <= lw - fw) bits[wi] = comp[k];
-      }
+  const int64_t win0 = static_cast<int64_t>(w) * (n + a.cp) + a.cp;
+  float2* zt = a.z + (ld * a.nt + t) * static_cast<int64_t>(a.words) * n + static_cast<int64_t>(w) * n;
+  for (int nl = tid; nl < n; nl += nth) {
+    const int64_t nabs = win0 + nl;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int p = 0; p < L; ++p) {
+      const int d = a.delays[p];
+      const int64_t m = nabs - d;
+      if (d < 0 || d > a.maxd || m < 0) continue;
+      const float2 h = taylor_eval(coef + (p * a.nb + static_cast<int>(tb.block_of(m))) * kTaylorCoeffs,
+                                   tb.offset(m));
+      acc = cfma(h, x[x0 + nl - d], acc);
     }
-    __syncthreads();
-    for (int q = tid; q < n; q += nth) {
-      const int el = q - a.active_lo;
-      x[x0 + subcarrier_bin(q, n, a.dc)] =
-          (el >= 0 && el < a.active) ? cons[symbol_bits_at(bits, bo + el * a.bps, a.bps)] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    lds_fft<+1>(x + x0, 1, a.xs, a.log2n, tw);
-    for (int i = tid; i < a.cpl; i += nth) x[x0 - a.cpl + i] = x[x0 + n - a.cpl + i];
-    __syncthreads();
-    // tapped delay line over the data window (the receiver keeps nothing else)
-    const int64_t win0 = static_cast<int64_t>(w) * period + a.cp;
-    for (int nl = tid; nl < n; nl += nth) {
-      const int64_t nabs = win0 + nl;
-      float2 acc = make_float2(0.f, 0.f);
-      for (int p = 0; p < L; ++p) {
-        const int d = a.delays[p];
-        const int64_t m = nabs - d;
-        if (d < 0 || d > a.maxd || m < 0) continue;
-        const float2 h = taylor_eval(coef + (p * a.nb + static_cast<int>(tb.block_of(m))) * kTaylorCoeffs,
-                                     tb.offset(m));
-        acc = cfma(h, x[x0 + nl - d], acc);
-      }
-      zt[static_cast<int64_t>(w) * n + nl] = acc;
-    }
-    // the last dx data samples feed the next word's delayed paths
-    float2 tail[kTail];
-#pragma unroll
-    for (int k = 0; k < kTail; ++k) {
-      const int i = tid + k * nth;
-      if (i < a.dx) tail[k] = x[x0 + n - a.dx + i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kTail; ++k) {
-      const int i = tid + k * nth;
-      if (i < a.dx) x[i] = tail[k];
-    }
-    __syncthreads();
+    zt[nl] = acc;
   }
 }
 
@@ -293,17 +325,20 @@ __global__ void __launch_bounds__(kRxThreads) massive_rx_kernel(MassiveArgs a) {
   float2* tw = reinterpret_cast<float2*>(lds + a.k2_tw);
   const int tid = threadIdx.x, nth = blockDim.x;
   const int groups = (a.nr + kRxGroup - 1) / kRxGroup;
-  const int64_t blk = blockIdx.x;
-  const int g = static_cast<int>(blk % groups);
-  const int w = static_cast<int>((blk / groups) % a.words);
-  const int64_t ld = blk / (static_cast<int64_t>(groups) * a.words);
+  // Workgroups b, b + 8, b + 16, ... share an XCD and its L2.  All antenna groups of one
+  // (drop, word) get block indices of one residue mod 8, so the users' delay-line outputs z
+  // they all read come from one L2: (drop, word) pair = x + 8 (q / groups), group = q % groups
+  // for b = x + 8 q.  The grid is padded to whole sets of 8 pairs.
+  const int64_t b = blockIdx.x;
+  const int64_t q = b / kXcds;
+  const int64_t pair = b % kXcds + kXcds * (q / groups);
+  if (pair >= a.num_pairs) return;
+  const int g = static_cast<int>(q % groups);
+  const int w = static_cast<int>(pair % a.words);
+  const int64_t ld = pair / a.words;
   const uint32_t drop = static_cast<uint32_t>(a.first_drop + ld);
   const int n = a.n, nt = a.nt;
-  for (int k = tid; k < n; k += nth) {
-    double sn, cs;
-    sincospi(-2.0 * static_cast<double>(k) / static_cast<double>(n), &sn, &cs);
-    tw[k] = make_float2(static_cast<float>(cs), static_cast<float>(sn));
-  }
+  fill_twiddles_octants(tw, n);
   const float sigma = static_cast<float>(sqrt(0.5 * a.noise_power[ld]));
   const float rx_scale = static_cast<float>(1.0 / sqrt(static_cast<double>(n)));
   const int64_t win0 = static_cast<int64_t>(w) * (n + a.cp) + a.cp;
@@ -314,16 +349,32 @@ __global__ void __launch_bounds__(kRxThreads) massive_rx_kernel(MassiveArgs a) {
   for (int r0 = g * kRxGroup; r0 < r_end; r0 += kRxRows) {
     const int rows = std::min(kRxRows, r_end - r0);
     __syncthreads();
-    for (int i = tid; i < rows * n; i += nth) {
-      const int rr = i / n, nl = i - rr * n;
+    // two samples per thread: one Philox call serves both noise samples of an aligned pair
+    const int half = n >> 1;
+    for (int i = tid; i < rows * half; i += nth) {
+      const int rr = i / half, nl = 2 * (i - rr * half);
       const int r = r0 + rr;
-      float2 acc = make_float2(0.f, 0.f);
-      for (int t = 0; t < nt; ++t) acc = cfma(sp[r * nt + t], zd[static_cast<int64_t>(t) * a.words * n + nl], acc);
+      float2 acc0 = make_float2(0.f, 0.f), acc1 = make_float2(0.f, 0.f);
+      for (int t = 0; t < nt; ++t) {
+        const float4 zz = *reinterpret_cast<const float4*>(zd + static_cast<int64_t>(t) * a.words * n + nl);
+        const float2 st = sp[r * nt + t];
+        acc0 = cfma(st, make_float2(zz.x, zz.y), acc0);
+        acc1 = cfma(st, make_float2(zz.z, zz.w), acc1);
+      }
       // noise sample nabs: words (0, 1) or (2, 3) of Philox call nabs / 2 (stream HPX_STREAM_NOISE + r)
       const int64_t nabs = win0 + nl;
-      const uint4 wn = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>(nabs >> 1));
-      const float2 nz = (nabs & 1) ? complex_normal_f32(wn.z, wn.w) : complex_normal_f32(wn.x, wn.y);
-      y[rr * n + nl] = make_float2(fmaf(sigma, nz.x, acc.x), fmaf(sigma, nz.y, acc.y));
+      float2 n0, n1;
+      const uint4 wa = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>(nabs >> 1));
+      if ((nabs & 1) == 0) {
+        n0 = complex_normal_f32(wa.x, wa.y);
+        n1 = complex_normal_f32(wa.z, wa.w);
+      } else {
+        const uint4 wb = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>((nabs + 1) >> 1));
+        n0 = complex_normal_f32(wa.z, wa.w);
+        n1 = complex_normal_f32(wb.x, wb.y);
+      }
+      y[rr * n + nl] = make_float2(fmaf(sigma, n0.x, acc0.x), fmaf(sigma, n0.y, acc0.y));
+      y[rr * n + nl + 1] = make_float2(fmaf(sigma, n1.x, acc1.x), fmaf(sigma, n1.y, acc1.y));
     }
     const bool csi_now = with_csi && r0 == g * kRxGroup;
     if (csi_now) {
@@ -368,8 +419,6 @@ constexpr int kDetK = 2 * kMaxRx;        // 128
 constexpr int kDetKs = kDetK + 4;        // padded LDS row
 
 __global__ void __launch_bounds__(kDetThreads) massive_detect_kernel(MassiveArgs a) {
-  __shared__ __align__(16) uint16_t yh[kDetElements * kDetKs];
-  __shared__ __align__(16) uint16_t yl[kDetElements * kDetKs];
   __shared__ __align__(16) uint16_t bh[32 * kDetKs];
   __shared__ __align__(16) uint16_t bl[32 * kDetKs];
   __shared__ float us[kDetElements][2 * kMaxUsers + 1];
@@ -402,40 +451,50 @@ __global__ void __launch_bounds__(kDetThreads) massive_detect_kernel(MassiveArgs
     bh[col * kDetKs + k] = h;
     bl[col * kDetKs + k] = f2bf(v - bf2f(h));
   }
-  // y tile: element e (row), k = r (re) or kMaxRx + r (im)
-  const float2* gy = a.grid + (ld * a.words + w) * static_cast<int64_t>(nr) * a.active;
-  for (int i = tid; i < kDetElements * kMaxRx; i += kDetThreads) {
-    const int r = i / kDetElements, e = i - r * kDetElements;  // consecutive threads: consecutive elements
-    const int k = k0 + e;
-    float2 v = make_float2(0.f, 0.f);
-    if (r < nr && k < a.active) v = gy[static_cast<int64_t>(r) * a.active + k];
-    uint16_t h = f2bf(v.x);
-    yh[e * kDetKs + r] = h;
-    yl[e * kDetKs + r] = f2bf(v.x - bf2f(h));
-    h = f2bf(v.y);
-    yh[e * kDetKs + kMaxRx + r] = h;
-    yl[e * kDetKs + kMaxRx + r] = f2bf(v.y - bf2f(h));
-  }
   __syncthreads();
   {
+    // A operand straight from the grid: lane (m, kq) of wave v holds element k0 + 32 v + m and
+    // k = 4 kq .. 4 kq + 3 of each 8-wide step; step s < 8 takes the real parts of antennas
+    // 8 s + 4 kq + j, step s + 8 their imaginary parts, so one load of 4 complex samples feeds
+    // two steps.  Consecutive lanes read consecutive elements of one antenna row.
     const int wave = tid >> 6, lane = tid & 63;
     const int m = lane & 31, kq = lane >> 5;
+    const int el = k0 + 32 * wave + m;
+    const bool live = el < a.active;
+    const float2* gy = a.grid + (ld * a.words + w) * static_cast<int64_t>(nr) * a.active + (live ? el : 0);
     floatx16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    const uint16_t* ah = yh + (32 * wave + m) * kDetKs + 4 * kq;
-    const uint16_t* al = yl + (32 * wave + m) * kDetKs + 4 * kq;
     const uint16_t* bhh = bh + m * kDetKs + 4 * kq;
     const uint16_t* bll = bl + m * kDetKs + 4 * kq;
-#pragma unroll 4
-    for (int k = 0; k < kDetK; k += 8) {
-      const bf16x4 a_h = *reinterpret_cast<const bf16x4*>(ah + k);
-      const bf16x4 a_l = *reinterpret_cast<const bf16x4*>(al + k);
-      const bf16x4 b_h = *reinterpret_cast<const bf16x4*>(bhh + k);
-      const bf16x4 b_l = *reinterpret_cast<const bf16x4*>(bll + k);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a_l, b_h, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a_h, b_l, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a_h, b_h, acc, 0, 0, 0);
+#pragma unroll 2
+    for (int st = 0; st < kMaxRx / 8; ++st) {
+      float2 yv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = 8 * st + 4 * kq + j;
+        yv[j] = live && r < nr ? gy[static_cast<int64_t>(r) * a.active] : make_float2(0.f, 0.f);
+      }
+      bf16x4 reh, rel, imh, iml;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint16_t hr = f2bf(yv[j].x), hi = f2bf(yv[j].y);
+        reh[j] = static_cast<short>(hr);
+        rel[j] = static_cast<short>(f2bf(yv[j].x - bf2f(hr)));
+        imh[j] = static_cast<short>(hi);
+        iml[j] = static_cast<short>(f2bf(yv[j].y - bf2f(hi)));
+      }
+      const int kr = 8 * st, ki = kMaxRx + 8 * st;
+      const bf16x4 brh = *reinterpret_cast<const bf16x4*>(bhh + kr);
+      const bf16x4 brl = *reinterpret_cast<const bf16x4*>(bll + kr);
+      const bf16x4 bih = *reinterpret_cast<const bf16x4*>(bhh + ki);
+      const bf16x4 bil = *reinterpret_cast<const bf16x4*>(bll + ki);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(rel, brh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(reh, brl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(reh, brh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(iml, bih, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(imh, bil, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(imh, bih, acc, 0, 0, 0);
     }
     // D[row][col]: row = element 32 wave + (r / 4) 8 + 4 kq + r % 4, col = m
     if (m < 2 * kMaxUsers) {
@@ -454,16 +513,23 @@ __global__ void __launch_bounds__(kDetThreads) massive_detect_kernel(MassiveArgs
     if (k >= a.active) continue;
     const float2 c = a.csi[(ld * a.words + w) * static_cast<int64_t>(a.active) + k];
     const float c2 = cabs2(c);
+    // fixed trip counts keep u in registers
     float2 u[kMaxUsers];
-    for (int i = 0; i < nt; ++i) {
-      const float den = fmaf(c2, dl[i], s2);
-      const float gi = (dl[i] > 0.0f || s2 > 0.0f) && den > 0.0f ? 1.0f / den : 0.0f;
-      u[i] = cscale(cmulc(c, make_float2(us[e][i], us[e][nt + i])), gi);
+#pragma unroll
+    for (int i = 0; i < kMaxUsers; ++i) {
+      u[i] = make_float2(0.f, 0.f);
+      if (i < nt) {
+        const float den = fmaf(c2, dl[i], s2);
+        const float gi = (dl[i] > 0.0f || s2 > 0.0f) && den > 0.0f ? 1.0f / den : 0.0f;
+        u[i] = cscale(cmulc(c, make_float2(us[e][i], us[e][nt + i])), gi);
+      }
     }
     const int64_t sym = static_cast<int64_t>(w) * a.active + k;
     for (int t = 0; t < nt; ++t) {
       float2 xh = make_float2(0.f, 0.f);
-      for (int i = 0; i < nt; ++i) xh = cfma(dv[t * nt + i], u[i], xh);
+#pragma unroll
+      for (int i = 0; i < kMaxUsers; ++i)
+        if (i < nt) xh = cfma(dv[t * nt + i], u[i], xh);
       const uint32_t rx = demap_symbol(a.mod, static_cast<double>(xh.x), static_cast<double>(xh.y));
       // transmitted bits of symbol sym of user t
       const int64_t pos = sym * a.bps;
@@ -544,7 +610,7 @@ int make_massive(const hpx_massive_link* l, MassiveArgs* out) {
   a.nb = static_cast<int>((a.frame_len + (int64_t(1) << a.log2b) - 1) >> a.log2b);
   a.maxd = l->max_delay;
   a.dx = std::max(0, l->max_delay - a.cp);
-  HPX_REQUIRE(a.dx <= kTail * kTxThreads && a.dx <= a.n, "hpx_massive_run: delay spread too long");
+  HPX_REQUIRE(a.dx <= a.n, "hpx_massive_run: delay spread too long");
   a.cpl = std::min(a.cp, a.maxd);
   a.x0 = a.dx + a.cpl;
   a.xs = a.x0 + a.n;
@@ -658,9 +724,12 @@ extern "C" int hpx_massive_run(const hpx_massive_link* link_host, const int32_t*
   }
   const unsigned nd = static_cast<unsigned>(num_drops);
   hipLaunchKernelGGL(massive_prologue_kernel, dim3(nd), dim3(256), a.k0_lds, s, a);
-  hipLaunchKernelGGL(massive_tx_kernel, dim3(nd * a.nt), dim3(kTxThreads), a.k1_lds, s, a);
-  const unsigned groups = static_cast<unsigned>((a.nr + kRxGroup - 1) / kRxGroup);
-  hipLaunchKernelGGL(massive_rx_kernel, dim3(nd * a.words * groups), dim3(kRxThreads), a.k2_lds, s, a);
+  hipLaunchKernelGGL(massive_tx_kernel, dim3(nd * a.nt * a.words), dim3(kTxThreads), a.k1_lds, s, a);
+  const int64_t groups = (a.nr + kRxGroup - 1) / kRxGroup;
+  a.num_pairs = static_cast<int64_t>(nd) * a.words;
+  const int64_t pair_sets = (a.num_pairs + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL(massive_rx_kernel, dim3(static_cast<unsigned>(pair_sets * kXcds * groups)), dim3(kRxThreads),
+                     a.k2_lds, s, a);
   const unsigned tiles = static_cast<unsigned>((a.active + kDetElements - 1) / kDetElements);
   hipLaunchKernelGGL(massive_detect_kernel, dim3(nd * a.words * tiles), dim3(kDetThreads), 0, s, a);
   return check_launch("hpx_massive_run");
