@@ -88,6 +88,84 @@ __host__ __device__ inline int hermitian_jacobi(int n, double* are, double* aim,
   return sweep;
 }
 
+// hermitian_jacobi executed by one wavefront: the same rotations in the same order, lane k
+// updating row k's entries of columns p, q (and of V), then, after a wavefront fence, row p's
+// and q's entries of column k.  Every entry sees exactly the arithmetic of the serial version,
+// so the results are identical; the serial dependency is one rotation, not one element.
+// Call from all 64 lanes of one wavefront (n <= 8; other lanes idle in the updates).
+__device__ inline int hermitian_jacobi_wave(int n, double* are, double* aim, double* vre, double* vim,
+                                            int max_sweeps = 30) {
+  const int k = static_cast<int>(threadIdx.x & 63);
+  auto sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  };
+  if (k < n)
+    for (int j = 0; j < n; ++j) {
+      vre[k * n + j] = (k == j) ? 1.0 : 0.0;
+      vim[k * n + j] = 0.0;
+    }
+  sync();
+  int sweep = 0;
+#pragma unroll 1
+  for (; sweep < max_sweeps; ++sweep) {
+    double off = 0.0, diag = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < n; ++i) {
+      diag += are[i * n + i] * are[i * n + i];
+#pragma unroll 1
+      for (int j = i + 1; j < n; ++j) off += are[i * n + j] * are[i * n + j] + aim[i * n + j] * aim[i * n + j];
+    }
+    if (off <= 1e-30 * diag || off == 0.0) break;
+#pragma unroll 1
+    for (int p = 0; p < n - 1; ++p)
+#pragma unroll 1
+      for (int q = p + 1; q < n; ++q) {
+        const double xr = are[p * n + q], xi = aim[p * n + q];
+        const double r = sqrt(xr * xr + xi * xi);
+        if (r == 0.0) continue;
+        const double er = xr / r, ei = xi / r;
+        const double tau = (are[q * n + q] - are[p * n + p]) / (2.0 * r);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+        sync();  // every lane has read the pivot before it changes
+        if (k < n) {
+          const double pr = are[k * n + p], pi = aim[k * n + p];
+          const double qr = are[k * n + q], qi = aim[k * n + q];
+          const double qer = qr * er + qi * ei, qei = qi * er - qr * ei;
+          are[k * n + p] = c * pr - s * qer;
+          aim[k * n + p] = c * pi - s * qei;
+          are[k * n + q] = s * pr + c * qer;
+          aim[k * n + q] = s * pi + c * qei;
+          const double vpr = vre[k * n + p], vpi = vim[k * n + p];
+          const double vqr = vre[k * n + q], vqi = vim[k * n + q];
+          const double vqer = vqr * er + vqi * ei, vqei = vqi * er - vqr * ei;
+          vre[k * n + p] = c * vpr - s * vqer;
+          vim[k * n + p] = c * vpi - s * vqei;
+          vre[k * n + q] = s * vpr + c * vqer;
+          vim[k * n + q] = s * vpi + c * vqei;
+        }
+        sync();
+        if (k < n) {
+          const double pr = are[p * n + k], pi = aim[p * n + k];
+          const double qr = are[q * n + k], qi = aim[q * n + k];
+          const double qer = qr * er - qi * ei, qei = qi * er + qr * ei;
+          are[p * n + k] = c * pr - s * qer;
+          aim[p * n + k] = c * pi - s * qei;
+          are[q * n + k] = s * pr + c * qer;
+          aim[q * n + k] = s * pi + c * qei;
+        }
+        sync();
+        if (k == 0) {
+          are[p * n + q] = aim[p * n + q] = are[q * n + p] = aim[q * n + p] = 0.0;
+          aim[p * n + p] = aim[q * n + q] = 0.0;
+        }
+        sync();
+      }
+  }
+  return sweep;
+}
+
 // Doubles of workspace spatial_detector needs.
 constexpr int kEigWork = 4 * kEigMax * kEigMax;
 

@@ -42,7 +42,7 @@ using namespace hpx;
 namespace {
 
 constexpr int kMaxRx = 64, kMaxUsers = 8;
-constexpr int kTxThreads = 512, kRxThreads = 512;
+constexpr int kTxThreads = 1024, kRxThreads = 1024;
 constexpr int kRxGroup = 8;      // antennas per receive workgroup
 constexpr int kRxRows = 2;       // antennas transformed together
 constexpr int kDetElements = 64;   // resource elements per detection workgroup (32 per wave)
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) massive_prologue_kernel(MassiveArgs a) {
     a.abar[ld * L * a.words + i] = make_float2(static_cast<float>(m.x), static_cast<float>(m.y));
   }
   // The per-drop detector of spatial_detector (hpx_eig.hpp) spread over the workgroup: the
-  // Gram matrix S^H S by nt * nt threads, its Jacobi eigen-decomposition by one thread,
+  // Gram matrix S^H S by nt * nt threads, its Jacobi eigen-decomposition by one wavefront,
   // A = V^H S^H by all threads (same sums in the same order).
   double* work = det + nt + 2 * nt * nt + 2 * nt * nr;
   double* gre = work;
@@ -206,8 +206,9 @@ __global__ void __launch_bounds__(256) massive_prologue_kernel(MassiveArgs a) {
     gim[e] = si;
   }
   __syncthreads();
+  if (tid < 64) hermitian_jacobi_wave(nt, gre, gim, vre, vim);
+  __syncthreads();
   if (tid == 0) {
-    hermitian_jacobi(nt, gre, gim, vre, vim);
     double lmax = 0.0;
     for (int i = 0; i < nt; ++i) lmax = fmax(lmax, gre[i * nt + i]);
     // numpy.linalg.pinv's cut: singular values below max(M, N) eps sigma_max are dropped
