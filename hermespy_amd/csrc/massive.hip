@@ -350,32 +350,43 @@ __global__ void __launch_bounds__(kRxThreads) massive_rx_kernel(MassiveArgs a) {
   for (int r0 = g * kRxGroup; r0 < r_end; r0 += kRxRows) {
     const int rows = std::min(kRxRows, r_end - r0);
     __syncthreads();
-    // two samples per thread: one Philox call serves both noise samples of an aligned pair
+    // two samples per thread and every row of the round at once: one load of the users' z feeds
+    // all rows, one Philox call serves both noise samples of an aligned pair
     const int half = n >> 1;
-    for (int i = tid; i < rows * half; i += nth) {
-      const int rr = i / half, nl = 2 * (i - rr * half);
-      const int r = r0 + rr;
-      float2 acc0 = make_float2(0.f, 0.f), acc1 = make_float2(0.f, 0.f);
+    for (int i = tid; i < half; i += nth) {
+      const int nl = 2 * i;
+      float2 acc[kRxRows][2];
+#pragma unroll
+      for (int rr = 0; rr < kRxRows; ++rr) acc[rr][0] = acc[rr][1] = make_float2(0.f, 0.f);
       for (int t = 0; t < nt; ++t) {
         const float4 zz = *reinterpret_cast<const float4*>(zd + static_cast<int64_t>(t) * a.words * n + nl);
-        const float2 st = sp[r * nt + t];
-        acc0 = cfma(st, make_float2(zz.x, zz.y), acc0);
-        acc1 = cfma(st, make_float2(zz.z, zz.w), acc1);
+#pragma unroll
+        for (int rr = 0; rr < kRxRows; ++rr) {
+          const float2 st = rr < rows ? sp[(r0 + rr) * nt + t] : make_float2(0.f, 0.f);
+          acc[rr][0] = cfma(st, make_float2(zz.x, zz.y), acc[rr][0]);
+          acc[rr][1] = cfma(st, make_float2(zz.z, zz.w), acc[rr][1]);
+        }
       }
-      // noise sample nabs: words (0, 1) or (2, 3) of Philox call nabs / 2 (stream HPX_STREAM_NOISE + r)
       const int64_t nabs = win0 + nl;
-      float2 n0, n1;
-      const uint4 wa = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>(nabs >> 1));
-      if ((nabs & 1) == 0) {
-        n0 = complex_normal_f32(wa.x, wa.y);
-        n1 = complex_normal_f32(wa.z, wa.w);
-      } else {
-        const uint4 wb = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>((nabs + 1) >> 1));
-        n0 = complex_normal_f32(wa.z, wa.w);
-        n1 = complex_normal_f32(wb.x, wb.y);
+#pragma unroll
+      for (int rr = 0; rr < kRxRows; ++rr) {
+        if (rr >= rows) break;
+        const int r = r0 + rr;
+        const float2 acc0 = acc[rr][0], acc1 = acc[rr][1];
+        // noise sample nabs: words (0, 1) or (2, 3) of Philox call nabs / 2 (stream HPX_STREAM_NOISE + r)
+        float2 n0, n1;
+        const uint4 wa = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>(nabs >> 1));
+        if ((nabs & 1) == 0) {
+          n0 = complex_normal_f32(wa.x, wa.y);
+          n1 = complex_normal_f32(wa.z, wa.w);
+        } else {
+          const uint4 wb = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>((nabs + 1) >> 1));
+          n0 = complex_normal_f32(wa.z, wa.w);
+          n1 = complex_normal_f32(wb.x, wb.y);
+        }
+        y[rr * n + nl] = make_float2(fmaf(sigma, n0.x, acc0.x), fmaf(sigma, n0.y, acc0.y));
+        y[rr * n + nl + 1] = make_float2(fmaf(sigma, n1.x, acc1.x), fmaf(sigma, n1.y, acc1.y));
       }
-      y[rr * n + nl] = make_float2(fmaf(sigma, n0.x, acc0.x), fmaf(sigma, n0.y, acc0.y));
-      y[rr * n + nl + 1] = make_float2(fmaf(sigma, n1.x, acc1.x), fmaf(sigma, n1.y, acc1.y));
     }
     const bool csi_now = with_csi && r0 == g * kRxGroup;
     if (csi_now) {
