@@ -81,7 +81,7 @@ struct BatchArgs {
   uint64_t seed;
   int64_t first_drop;
   // LDS byte offsets
-  int off_x, off_coef, off_abar, off_tw, off_bits, off_cons, off_s, off_det, off_red, off_tmp_sos, off_tmp_s64;
+  int off_x, off_coef, off_abar, off_tw, off_bits, off_cons, off_s, off_det, off_red, off_dly, off_tmp_sos, off_tmp_s64;
 };
 
 // bps bits starting at bit `bit` of an LSB-first word window (bit i of the result = i-th bit)
@@ -105,6 +105,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
   float2* cons = reinterpret_cast<float2*>(lds + a.off_cons);
   float2* smat = reinterpret_cast<float2*>(lds + a.off_s);
   int* red = reinterpret_cast<int*>(lds + a.off_red);
+  int* dly = reinterpret_cast<int*>(lds + a.off_dly);  // path delays: LDS broadcasts in the channel loop
   constexpr int kRows = NT > NR ? NT : NR;
   float2* csirow = xbuf + kRows * a.xs + a.x0;                     // [N] per-word CSI over the taps
   double* uni = reinterpret_cast<double*>(lds + a.off_x);        // prologue only
@@ -134,6 +135,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     }
   }
   if (tid == 0) red[0] = 0;
+  for (int p = tid; p < L; p += nthreads) dly[p] = a.delays[p];
   __syncthreads();
 
   // spatial response S = R_rx exp(2 pi j U)[:NR, :NT] R_tx (fading.py:476-489)
@@ -285,7 +287,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     }
     if (tid == 0) {
       for (int p = 0; p < L; ++p) {
-        const int d = a.delays[p];
+        const int d = dly[p];
         if (d >= 0 && d < a.keep_below) csirow[d] = cadd(csirow[d], abar[p * a.words + w]);
       }
     }
@@ -307,7 +309,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
         if (win0 >= a.maxd) {
           // no delayed index reaches before the frame start: no per-lane checks
           for (int p = 0; p < L; ++p) {
-            const int d = a.delays[p];
+            const int d = dly[p];
             if (d < 0 || d > a.maxd) continue;
             const int m = static_cast<int>(nabs) - d;
             const int b0 = m >> a.log2b;
@@ -326,7 +328,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
           }
         } else {
           for (int p = 0; p < L; ++p) {
-            const int d = a.delays[p];
+            const int d = dly[p];
             if (d < 0 || d > a.maxd) continue;
             const int64_t m = nabs - d;
             const float2* xc = xbuf + x0 + nl - d;
@@ -626,6 +628,8 @@ int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out) {
   o += align16(static_cast<int64_t>(nt * nt + nt * nr) * sizeof(float2) + nt * sizeof(float));
   a.off_red = static_cast<int>(o);
   o += 16;
+  a.off_dly = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.paths) * sizeof(int32_t));
   out->lds = o;
   if (o > 160 * 1024) {
     set_error("hpx_run_batch: link needs %lld bytes of LDS per drop (max 163840)", static_cast<long long>(o));

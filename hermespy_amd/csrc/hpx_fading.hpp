@@ -84,12 +84,12 @@ struct TaylorBlocks {
     return static_cast<double>(b << log2b) + 0.5 * static_cast<double>((1 << log2b) - 1);
   }
   __device__ __forceinline__ double half_width() const { return 0.5 * static_cast<double>(1 << log2b); }
-  // normalised offset of sample m in its block, exact in float32
+  // normalised offset (2 r - (B - 1)) / B of sample m in its block (r = m mod B), exact in
+  // float32; B is a power of two, so the division is an exact scaling by 2^-log2b
   __device__ __forceinline__ float offset(int64_t m) const {
-    const int64_t b = m >> log2b;
-    const int64_t r = m - (b << log2b);  // 0 .. B-1
+    const int r = static_cast<int>(m & ((int64_t(1) << log2b) - 1));  // 0 .. B-1
     const float B = static_cast<float>(1 << log2b);
-    return (2.0f * static_cast<float>(r) - (B - 1.0f)) / B;
+    return ldexpf(2.0f * static_cast<float>(r) - (B - 1.0f), -log2b);
   }
 };
 

@@ -81,7 +81,7 @@ struct MassiveArgs {
   int32_t* errors;  // [drop], accumulated
   float2* equalized;  // [drop][nt][words][active] or null
   int k0_uni, k0_sos, k0_e, k0_f, k0_s64, k0_det, k0_lds;
-  int k1_x, k1_tw, k1_coef, k1_bits, k1_cons, k1_lds;
+  int k1_x, k1_tw, k1_coef, k1_bits, k1_cons, k1_dly, k1_lds;
   int k2_y, k2_tw, k2_lds;
 };
 
@@ -273,6 +273,7 @@ __global__ void __launch_bounds__(kTxThreads) massive_tx_kernel(MassiveArgs a) {
   float2* coef = reinterpret_cast<float2*>(lds + a.k1_coef);
   uint32_t* bits = reinterpret_cast<uint32_t*>(lds + a.k1_bits);
   float2* cons = reinterpret_cast<float2*>(lds + a.k1_cons);
+  int* dly = reinterpret_cast<int*>(lds + a.k1_dly);  // path delays: LDS broadcasts in the delay line
   const int tid = threadIdx.x, nth = blockDim.x;
   const int64_t blk = blockIdx.x;
   const int w = static_cast<int>(blk % a.words);
@@ -283,6 +284,7 @@ __global__ void __launch_bounds__(kTxThreads) massive_tx_kernel(MassiveArgs a) {
   const int n = a.n, L = a.paths, x0 = a.x0;
   const int ncoef = L * a.nb * kTaylorCoeffs;
   for (int i = tid; i < ncoef; i += nth) coef[i] = a.coef[ld * ncoef + i];
+  for (int p = tid; p < L; p += nth) dly[p] = a.delays[p];
   for (int v = tid; v < a.order; v += nth) {
     const double2 p = constellation_point(a.order, 1, bits_to_label(static_cast<uint32_t>(v), a.bps));
     cons[v] = make_float2(static_cast<float>(p.x * a.tx_scale), static_cast<float>(p.y * a.tx_scale));
@@ -308,7 +310,7 @@ __global__ void __launch_bounds__(kTxThreads) massive_tx_kernel(MassiveArgs a) {
     const int64_t nabs = win0 + nl;
     float2 acc = make_float2(0.f, 0.f);
     for (int p = 0; p < L; ++p) {
-      const int d = a.delays[p];
+      const int d = dly[p];
       const int64_t m = nabs - d;
       if (d < 0 || d > a.maxd || m < 0) continue;
       const float2 h = taylor_eval(coef + (p * a.nb + static_cast<int>(tb.block_of(m))) * kTaylorCoeffs,
@@ -655,6 +657,8 @@ int make_massive(const hpx_massive_link* l, MassiveArgs* out) {
   o += align16(static_cast<int64_t>(a.bits_words) * 4);
   a.k1_cons = static_cast<int>(o);
   o += align16(static_cast<int64_t>(a.order) * 8);
+  a.k1_dly = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.paths) * 4);
   a.k1_lds = static_cast<int>(o);
   // K2 LDS
   a.k2_y = 0;
