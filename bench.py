@@ -24,8 +24,8 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3  # published MI355X FP32 peak (packed FP32 math)
 FP32_UNPACKED_PEAK_TFLOPS = 78.6  # one FP32 FMA per lane and cycle: the kernels are built without packed FP32
 # HBM bytes of one fused 4x4 drop, from rocprofv3 FETCH_SIZE / WRITE_SIZE over 4096-drop launches
 # (FETCH x2.0 and WRITE x1.0, the factors a 256 MiB device copy reads in the same process):
-# (4876.7 x 2 + 71371.7) KiB / 4096; almost all of it is register-spill stores
-HBM_BYTES_PER_DROP = (4876.66 * 2.0 + 71371.69) * 1024.0 / 4096.0
+# (1783.5 x 2 + 15626.1) KiB / 4096; almost all of it is register-spill stores
+HBM_BYTES_PER_DROP = (1783.52 * 2.0 + 15626.06) * 1024.0 / 4096.0
 DROPS_PER_STEP = 4096
 SEED = 20261202
 
