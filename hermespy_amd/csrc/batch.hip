@@ -198,24 +198,33 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
 
   // Taylor coefficients of every (path, block) and word means of every (path, word)
   const TaylorBlocks tb{a.log2b};
-  if (tid == nthreads - 1) {
-    // per-drop detector of the rank-one-in-frequency ideal-CSI channel, on an LDS workspace
+  // The last wavefront computes the per-drop detector of the rank-one-in-frequency ideal-CSI
+  // channel on an LDS workspace while the others compute the Taylor coefficients (one
+  // wavefront does both in turn).
+  const bool split = nthreads >= 128;
+  const int det_first = split ? nthreads - 64 : 0;
+  if (tid >= det_first) {
     double* lam = s64 + 2 * NR * NT;
     double* v = lam + NT;
     double* am = v + 2 * NT * NT;
-    spatial_detector(NR, NT, s64, lam, v, am, am + 2 * NT * NR);
-    for (int i = 0; i < NT * NT; ++i) det_v[i] = make_float2(static_cast<float>(v[2 * i]), static_cast<float>(v[2 * i + 1]));
-    for (int i = 0; i < NT * NR; ++i)
+    spatial_detector_wave(NR, NT, s64, lam, v, am, am + 2 * NT * NR);
+    const int lane = tid - det_first;
+    for (int i = lane; i < NT * NT; i += 64)
+      det_v[i] = make_float2(static_cast<float>(v[2 * i]), static_cast<float>(v[2 * i + 1]));
+    for (int i = lane; i < NT * NR; i += 64)
       det_a[i] = make_float2(static_cast<float>(am[2 * i]), static_cast<float>(am[2 * i + 1]));
     double lmax = 0.0;
     for (int i = 0; i < NT; ++i) lmax = fmax(lmax, lam[i]);
     // numpy.linalg.pinv's cut: singular values below max(M, N) eps sigma_max are dropped
     const double rt = static_cast<double>(NR > NT ? NR : NT) * 2.220446049250313e-16;
-    for (int i = 0; i < NT; ++i) det_l[i] = lam[i] > rt * rt * lmax ? static_cast<float>(lam[i]) : 0.0f;
+    for (int i = lane; i < NT; i += 64) det_l[i] = lam[i] > rt * rt * lmax ? static_cast<float>(lam[i]) : 0.0f;
   }
-  for (int e = tid; e < L * a.nb; e += nthreads) {
-    const int p = e / a.nb, b = e - p * a.nb;
-    taylor_coeffs(sos + 3 * p * a.terms, a.terms, tb.centre(b), tb.half_width(), coef + e * kTaylorCoeffs);
+  const int taylor_threads = split ? det_first : nthreads;
+  if (tid < taylor_threads) {
+    for (int e = tid; e < L * a.nb; e += taylor_threads) {
+      const int p = e / a.nb, b = e - p * a.nb;
+      taylor_coeffs(sos + 3 * p * a.terms, a.terms, tb.centre(b), tb.half_width(), coef + e * kTaylorCoeffs);
+    }
   }
   for (int e = tid; e < L * a.words; e += nthreads) {
     const int p = e / a.words, w = e - p * a.words;

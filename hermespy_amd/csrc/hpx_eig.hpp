@@ -220,4 +220,54 @@ __host__ __device__ inline void spatial_detector(int nr, int nt, const double* s
     }
 }
 
+
+// spatial_detector executed by one wavefront (all 64 lanes call it; the same sums in the
+// same order, so the results equal the serial version's): the Gram matrix and A = V^H S^H
+// spread over the lanes, the Jacobi rotations by hermitian_jacobi_wave.
+__device__ inline void spatial_detector_wave(int nr, int nt, const double* s_ri, double* lambda, double* v_ri,
+                                             double* a_ri, double* work) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  auto sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  };
+  double* gre = work;
+  double* gim = work + kEigMax * kEigMax;
+  double* vre = work + 2 * kEigMax * kEigMax;
+  double* vim = work + 3 * kEigMax * kEigMax;
+  for (int e = lane; e < nt * nt; e += 64) {
+    const int i = e / nt, j = e - i * nt;
+    double sr = 0.0, si = 0.0;
+    for (int r = 0; r < nr; ++r) {  // conj(S_ri) S_rj
+      const double ar = s_ri[2 * (r * nt + i)], ai = s_ri[2 * (r * nt + i) + 1];
+      const double br = s_ri[2 * (r * nt + j)], bi = s_ri[2 * (r * nt + j) + 1];
+      sr += ar * br + ai * bi;
+      si += ar * bi - ai * br;
+    }
+    gre[e] = sr;
+    gim[e] = si;
+  }
+  sync();
+  hermitian_jacobi_wave(nt, gre, gim, vre, vim);
+  sync();
+  for (int i = lane; i < nt; i += 64) lambda[i] = gre[i * nt + i];
+  for (int i = lane; i < nt * nt; i += 64) {
+    v_ri[2 * i] = vre[i];
+    v_ri[2 * i + 1] = vim[i];
+  }
+  for (int e = lane; e < nt * nr; e += 64) {
+    const int i = e / nr, r = e - i * nr;
+    double ar = 0.0, ai = 0.0;
+    for (int k = 0; k < nt; ++k) {  // A[i][r] = sum_k conj(V[k][i]) conj(S[r][k])
+      const double vr = vre[k * nt + i], vi = -vim[k * nt + i];
+      const double sr = s_ri[2 * (r * nt + k)], si = -s_ri[2 * (r * nt + k) + 1];
+      ar += vr * sr - vi * si;
+      ai += vr * si + vi * sr;
+    }
+    a_ri[2 * (i * nr + r)] = ar;
+    a_ri[2 * (i * nr + r) + 1] = ai;
+  }
+  sync();
+}
+
 }  // namespace hpx
