@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(256) weight_copies_kernel(const float* __restr
 // the output written straight into the estimate grid.  Numerically the layer-by-layer path: the
 // same float32 accumulation, bias, GELU and bfloat16 rounding between layers; the 64-channel
 // layers use the 32x32x16 MFMA (8 channels per lane and step).
-constexpr int kFusedSamples = 2;
+constexpr int kFusedSamples = 4;  // samples per workgroup: 78 KB of LDS, two workgroups per CU
 constexpr int kS8 = 8 + 4, kS64 = 64 + 8;  // padded LDS rows
 
 // 8-channel input (32x32x8 steps): tap k reads row (p + (k - TAPS/2) dil) mod 64
