@@ -270,7 +270,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     __syncthreads();
     // map + place at the IFFT bins; clear the CSI row (row kRows: tap means at their delays)
     for (int e = tid; e < NT * n; e += nthreads) {
-      const int t = e / n, q = e - t * n;
+      const int t = e >> a.log2n, q = e & (n - 1);  // n is a power of two
       const int el = a.elements ? a.elements[w * n + q] : q;
       float2 v;
       if (el >= 0) {
