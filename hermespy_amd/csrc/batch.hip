@@ -269,22 +269,27 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     }
     __syncthreads();
     // map + place at the IFFT bins; clear the CSI row (row kRows: tap means at their delays)
-    for (int e = tid; e < NT * n; e += nthreads) {
-      const int t = e >> a.log2n, q = e & (n - 1);  // n is a power of two
+    // (element map, bin and bit offset once per subcarrier for all streams)
+    for (int q = tid; q < n; q += nthreads) {
       const int el = a.elements ? a.elements[w * n + q] : q;
-      float2 v;
-      if (el >= 0) {
-        v = cons[symbol_bits(bits + t * a.bits_words, bo + el * a.bps, a.bps)];
-      } else if (el == HPX_ELEMENT_NULL) {
-        v = make_float2(0.f, 0.f);
-      } else {
-        const float2 pv = a.pilots[t * a.nref + (-1 - el)];
-        v = make_float2(static_cast<float>(static_cast<double>(pv.x) * a.tx_scale),
-                        static_cast<float>(static_cast<double>(pv.y) * a.tx_scale));
+      const int bin = subcarrier_bin(q, n, a.dc);
+      const int bit = bo + el * a.bps;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        float2 v;
+        if (el >= 0) {
+          v = cons[symbol_bits(bits + t * a.bits_words, bit, a.bps)];
+        } else if (el == HPX_ELEMENT_NULL) {
+          v = make_float2(0.f, 0.f);
+        } else {
+          const float2 pv = a.pilots[t * a.nref + (-1 - el)];
+          v = make_float2(static_cast<float>(static_cast<double>(pv.x) * a.tx_scale),
+                          static_cast<float>(static_cast<double>(pv.y) * a.tx_scale));
+        }
+        xbuf[t * a.xs + x0 + bin] = v;
       }
-      xbuf[t * a.xs + x0 + subcarrier_bin(q, n, a.dc)] = v;
+      csirow[q] = make_float2(0.f, 0.f);
     }
-    for (int q = tid; q < n; q += nthreads) csirow[q] = make_float2(0.f, 0.f);
     __syncthreads();
     lds_fft<+1, FFTV>(xbuf + x0, NT, a.xs, a.log2n, tw);
     // cyclic prefix; word means of the taps at their delays (paths sharing a delay add up,
