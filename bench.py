@@ -24,8 +24,8 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3  # published MI355X FP32 peak (packed FP32 math)
 FP32_UNPACKED_PEAK_TFLOPS = 78.6  # one FP32 FMA per lane and cycle: the kernels are built without packed FP32
 # HBM bytes of one fused 4x4 drop, from rocprofv3 FETCH_SIZE / WRITE_SIZE over 4096-drop launches
 # (FETCH x2.0 and WRITE x1.0, the factors a 256 MiB device copy reads in the same process):
-# (1783.5 x 2 + 15626.1) KiB / 4096; almost all of it is register-spill stores
-HBM_BYTES_PER_DROP = (1783.52 * 2.0 + 15626.06) * 1024.0 / 4096.0
+# (1797.0 x 2 + 15695.0) KiB / 4096; almost all of it is register-spill stores
+HBM_BYTES_PER_DROP = (1797.0 * 2.0 + 15695.0) * 1024.0 / 4096.0
 DROPS_PER_STEP = 4096
 SEED = 20261202
 
@@ -219,6 +219,37 @@ def uneven_leg(drops: int = 8192, reps: int = 5) -> dict:
             "workgroup_choice": tuning["choice"]}
 
 
+def campaign_leg(num_samples: int = 2048) -> dict:
+    """The 4x4 link's BER campaign over the 11 SNR sections through BatchedSimulation.simulate:
+    section queue, indexed launches of up to 16 384 drops, device section statistics and the
+    per-launch read-back of the stopping test, without stopping (default evaluator) and with
+    the reference's rule at tolerance 1e-3, confidence 0.05, 1024 samples between tests."""
+
+    import time
+
+    import torch
+
+    from hermespy_amd.configs import mimo4x4_1024
+    from hermespy_amd.modem import BitErrorEvaluator
+    from hermespy_amd.simulation import BatchedSimulation
+
+    sim = BatchedSimulation(mimo4x4_1024(), seed=SEED)
+    sim.simulate(64)  # warm-up
+    out = {"workload": "mimo4x4-1024sc-16qam-mmse, 11 SNR sections", "num_samples": num_samples}
+    for name, ev in (("full", None), ("early_stopping", BitErrorEvaluator(confidence=0.05, tolerance=1e-3,
+                                                                         min_num_samples=1024))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = sim.simulate(num_samples, ev)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        drops = int(res.evaluation_results[0].num_drops().sum())
+        out[name] = {"drops": drops, "seconds": wall, "drops_per_s": drops / wall,
+                     "drops_per_section": res.evaluation_results[0].num_drops().ravel().tolist(),
+                     "ber": [float(x) for x in res.evaluation_results[0].to_array().ravel()]}
+    return out
+
+
 def massive_leg(drops: int = 64, chunk: int = 32) -> dict:
     """64x8 massive-MIMO uplink drops (4096-point OFDM, 3276 active subcarriers, 256-QAM, LMMSE)."""
 
@@ -261,6 +292,7 @@ def main() -> int:
     ap.add_argument("--no-estimators", action="store_true", help="skip the pilot-estimation leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the heterogeneous-batch leg")
     ap.add_argument("--no-massive", action="store_true", help="skip the massive-MIMO leg")
+    ap.add_argument("--no-campaign", action="store_true", help="skip the early-stopping campaign leg")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (counter calibration for profiles)")
     args = ap.parse_args()
@@ -350,6 +382,8 @@ def main() -> int:
         out["uneven_batch"] = uneven_leg()
     if not args.no_massive:
         out["massive_mimo"] = massive_leg()
+    if not args.no_campaign:
+        out["campaign"] = campaign_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
     print(json.dumps(out), flush=True)
