@@ -81,7 +81,7 @@ class UnevenBatch:
     first_drop + k)."""
 
     def __init__(self, shapes: list, seed: int = 20261202, first_drop: int = 0, device="cuda",
-                 workgroup_sizes: dict | None = None) -> None:
+                 workgroup_sizes: dict | None = None, streams: int = 4) -> None:
         self.shapes = shapes
         self.seed = int(seed)
         self.first_drop = int(first_drop)
@@ -105,6 +105,9 @@ class UnevenBatch:
                                            dtype=torch.float64, device=dev)
         self.errors = torch.zeros(n, dtype=torch.int32, device=dev)
         self.bits = np.array([c.bits_per_drop for c in self.links], dtype=np.int64)
+        # Classes of a few hundred drops each fill a fraction of the GPU; launched on several
+        # streams (default priority) they run side by side, also inside a captured graph.
+        self.streams = [torch.cuda.Stream(device=dev) for _ in range(max(1, int(streams)))]
         self.prepare(workgroup_sizes or {})
 
     def class_key(self, i: int) -> tuple:
@@ -133,8 +136,26 @@ class UnevenBatch:
         self.graph = None
 
     def run(self) -> None:
-        _lib.check(self.lib.hpx_uneven_run(self.plans.data_ptr(), self.classes, self.num_classes,
-                                           _lib.stream_handle()))
+        """All classes, asynchronous on the current stream: forked over the side streams in class
+        order (heaviest first, round robin) and joined back."""
+
+        cur = torch.cuda.current_stream(self.device)
+        if len(self.streams) <= 1 or self.num_classes <= 1:
+            _lib.check(self.lib.hpx_uneven_run(self.plans.data_ptr(), self.classes, self.num_classes,
+                                               cur.cuda_stream))
+            return
+        fork = torch.cuda.Event()
+        fork.record(cur)
+        for st in self.streams:
+            st.wait_event(fork)
+        for k in range(self.num_classes):
+            st = self.streams[k % len(self.streams)]
+            _lib.check(self.lib.hpx_uneven_run(self.plans.data_ptr(), ctypes.byref(self.classes[k]), 1,
+                                               st.cuda_stream))
+        for st in self.streams:
+            join = torch.cuda.Event()
+            join.record(st)
+            cur.wait_event(join)
 
     def run_class(self, k: int) -> None:
         _lib.check(self.lib.hpx_uneven_run(self.plans.data_ptr(), ctypes.byref(self.classes[k]), 1,

@@ -110,6 +110,11 @@ def test_uneven_graph_replay_matches_single_link_launches(cuda):
         torch.cuda.synchronize()
         assert int(err.item()) == int(eager[i]), (i, pop[i])
     assert 0 < eager.sum()
+    # the classes forked over side streams give what one stream gives
+    one = UnevenBatch(pop, seed=99, first_drop=1000, streams=1)
+    one.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(one.errors.cpu().numpy(), eager)
 
 
 @pytest.mark.gpu
