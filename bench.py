@@ -250,7 +250,7 @@ def campaign_leg(num_samples: int = 2048) -> dict:
     return out
 
 
-def massive_leg(drops: int = 64, chunk: int = 32) -> dict:
+def massive_leg(drops: int = 256, chunk: int = 128) -> dict:
     """64x8 massive-MIMO uplink drops (4096-point OFDM, 3276 active subcarriers, 256-QAM, LMMSE)."""
 
     import torch
