@@ -402,5 +402,6 @@ def run_campaign(batched, seed: int, snr_db: Sequence[float], num_samples: int, 
         stop = result.stopping_test(tested, counters)
         for s in tested[stop]:
             queue.disable_section(int(s))
-    torch.cuda.synchronize(dev) if dev.type == "cuda" else None
+    if dev.type == "cuda":
+        torch.cuda.synchronize(dev)
     return MonteCarloResult(grid, [ev], [result], time.perf_counter() - start)
