@@ -108,3 +108,30 @@ def test_host_spatial_detector(native_lib, nr, nt):
         h = (c * s)[:, :, None]
         ref = equalize.mmse(y[:, None], h, s2)[:, 0] if s2 > 0 else equalize.zero_forcing(y[:, None], h)[:, 0]
         np.testing.assert_allclose(x, ref, rtol=1e-9, atol=1e-12)
+
+
+def test_new_entry_points_validate_without_a_device(native_lib):
+    """Argument checks of the drop-index / recorded-normals and section-statistics entry points
+    (no device call happens when they fail)."""
+    import ctypes
+
+    from hermespy_amd import _lib
+
+    stats = _lib.HpxSectionStats()
+    with pytest.raises(ValueError, match="sections"):
+        _lib.check(native_lib.hpx_section_accumulate(None, None, 0, None, 1, 10, 0, ctypes.byref(stats), None))
+    with pytest.raises(ValueError, match="bits per drop"):
+        _lib.check(native_lib.hpx_section_accumulate(None, None, 0, None, 0, 10, 4, ctypes.byref(stats), None))
+    with pytest.raises(ValueError, match="null"):
+        _lib.check(native_lib.hpx_section_accumulate(None, None, 0, None, 8, 10, 4, ctypes.byref(stats), None))
+    assert native_lib.hpx_section_accumulate(None, None, 0, None, 8, 0, 4, ctypes.byref(stats), None) == 0
+    with pytest.raises(ValueError, match="65535"):
+        _lib.check(native_lib.hpx_realization_normals_batch(1, None, 0, 70000, 100, None, None))
+    with pytest.raises(ValueError, match="32 bits"):
+        _lib.check(native_lib.hpx_realization_normals_batch(1, None, 2**32 - 1, 2, 100, None, None))
+    assert native_lib.hpx_realization_normals_batch(1, None, 0, 0, 100, None, None) == 0
+    link = _lib.HpxLink()
+    out = _lib.HpxBatchOut()
+    with pytest.raises(ValueError, match="null input or output"):
+        _lib.check(native_lib.hpx_run_batch_drops(ctypes.byref(link), None, None, None, None, None, None, 1, 0, 1,
+                                                  ctypes.byref(out), None))
