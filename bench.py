@@ -214,7 +214,8 @@ def uneven_leg(drops: int = 8192, reps: int = 5) -> dict:
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / reps
     bits = float(ub.bits.sum())
-    return {"workload": "uneven mix (8x8 at 2048 subcarriers excluded)", "drops": drops, "ms_per_launch": ms,
+    return {"workload": "uneven mix (8x8 at 2048 subcarriers in lean classes)", "drops": drops,
+            "lean_drops": ub.num_lean, "ms_per_launch": ms,
             "drops_per_s": drops / (ms / 1e3), "bits_per_s": bits / (ms / 1e3), "classes": ub.num_classes,
             "workgroup_choice": tuning["choice"]}
 

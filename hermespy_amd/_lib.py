@@ -16,7 +16,7 @@ import torch  # noqa: F401  (must precede the library load, see module docstring
 
 LIB_PATH = Path(__file__).resolve().parent / "_native" / "libhermespy_amd.so"
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 HPX_OK = 0
 HPX_ERR_INVALID = -1
@@ -86,14 +86,14 @@ class HpxSectionStats(ctypes.Structure):
 class HpxDropTables(ctypes.Structure):
     """Mirror of ``hpx_drop_tables`` (device pointers)."""
 
-    _fields_ = [("delays", c_vp), ("path_gains", c_vp), ("corr_rx", c_vp), ("corr_tx", c_vp)]
+    _fields_ = [("delays", c_vp), ("path_gains", c_vp), ("corr_rx", c_vp), ("corr_tx", c_vp), ("workspace", c_vp)]
 
 
 class HpxUnevenClass(ctypes.Structure):
     """Mirror of ``hpx_uneven_class``."""
 
     _fields_ = [("first", c_i64), ("count", c_i64), ("num_tx", c_i32), ("threads", c_i32), ("pairs", c_i32),
-                ("reserved", c_i32), ("lds_bytes", c_i64)]
+                ("lean", c_i32), ("lds_bytes", c_i64)]
 
 
 class HpxMassiveLink(ctypes.Structure):
@@ -134,6 +134,7 @@ SIGNATURES = {
     "hpx_run_batch_frames": (c_int, [ctypes.POINTER(HpxLink), ctypes.POINTER(HpxFrame), c_vp, c_vp, c_vp, c_vp,
                                      c_vp, c_u64, c_i64, c_i64, c_vp, ctypes.POINTER(HpxBatchOut), c_vp]),
     "hpx_uneven_plan_bytes": (c_i64, []),
+    "hpx_uneven_workspace_elements": (c_i64, [ctypes.POINTER(HpxLink)]),
     "hpx_uneven_prepare": (c_int, [ctypes.POINTER(HpxLink), ctypes.POINTER(HpxDropTables), c_vp, c_i64, c_u64, c_vp,
                                    c_vp, c_vp, c_vp, ctypes.POINTER(HpxUnevenClass), c_int, ctypes.POINTER(c_int)]),
     "hpx_uneven_run": (c_int, [c_vp, ctypes.POINTER(HpxUnevenClass), c_int, c_vp]),
@@ -211,6 +212,14 @@ def check(status: int) -> None:
     if status in (HPX_ERR_INVALID, HPX_ERR_UNSUPPORTED):
         raise ValueError(msg)
     raise RuntimeError(msg)
+
+
+def check_count(value: int) -> int:
+    """A count returned by the library, or its error (negative counts) raised as ValueError."""
+
+    if value < 0:
+        raise ValueError(last_error())
+    return int(value)
 
 
 def ptr(t) -> int | None:

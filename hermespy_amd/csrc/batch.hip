@@ -82,6 +82,9 @@ struct BatchArgs {
   int64_t first_drop;
   // LDS byte offsets
   int off_x, off_coef, off_abar, off_tw, off_bits, off_cons, off_s, off_det, off_red, off_dly, off_tmp_sos, off_tmp_s64;
+  // lean drops (working set beyond one CU's LDS): Taylor table and twiddles in device memory
+  float2* ws_coef;
+  float2* ws_tw;
 };
 
 // bps bits starting at bit `bit` of an LSB-first word window (bit i of the result = i-th bit)
@@ -94,20 +97,21 @@ __device__ __forceinline__ uint32_t symbol_bits(const uint32_t* bw, int bit, int
 // One drop, simulated by the whole workgroup.  local_drop indexes the per-drop arrays
 // (noise power, Doppler, errors, exports); the Philox drop index is drop_index[local_drop]
 // or first_drop + local_drop.
-template <int NT, int NR, int PAIRS, int FFTV>
+template <int NT, int NR, int PAIRS, int FFTV, bool LEAN = false>
 __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t local_drop) {
   extern __shared__ __align__(16) unsigned char lds[];
   float2* xbuf = reinterpret_cast<float2*>(lds + a.off_x);
-  float2* coef = reinterpret_cast<float2*>(lds + a.off_coef);
+  // a lean drop keeps the Taylor table and the twiddles in its device workspace
+  float2* coef = LEAN ? a.ws_coef : reinterpret_cast<float2*>(lds + a.off_coef);
   float2* abar = reinterpret_cast<float2*>(lds + a.off_abar);
-  float2* tw = reinterpret_cast<float2*>(lds + a.off_tw);
+  float2* tw = LEAN ? a.ws_tw : reinterpret_cast<float2*>(lds + a.off_tw);
   uint32_t* bits = reinterpret_cast<uint32_t*>(lds + a.off_bits);
   float2* cons = reinterpret_cast<float2*>(lds + a.off_cons);
   float2* smat = reinterpret_cast<float2*>(lds + a.off_s);
   int* red = reinterpret_cast<int*>(lds + a.off_red);
   int* dly = reinterpret_cast<int*>(lds + a.off_dly);  // path delays: LDS broadcasts in the channel loop
   constexpr int kRows = NT > NR ? NT : NR;
-  float2* csirow = xbuf + kRows * a.xs + a.x0;                     // [N] per-word CSI over the taps
+  float2* csirow = xbuf + kRows * a.xs + a.x0;  // [N] per-word CSI over the taps (not for lean drops)
   double* uni = reinterpret_cast<double*>(lds + a.off_x);        // prologue only
   double* sos = reinterpret_cast<double*>(lds + a.off_tmp_sos);  // prologue only
   double* s64 = reinterpret_cast<double*>(lds + a.off_tmp_s64);  // prologue only: S in float64
@@ -288,7 +292,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
         }
         xbuf[t * a.xs + x0 + bin] = v;
       }
-      csirow[q] = make_float2(0.f, 0.f);
+      if (!LEAN) csirow[q] = make_float2(0.f, 0.f);
     }
     __syncthreads();
     lds_fft<+1, FFTV>(xbuf + x0, NT, a.xs, a.log2n, tw);
@@ -299,7 +303,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
       const int t = e / a.cpl, i = e - t * a.cpl;
       xbuf[t * a.xs + x0 - a.cpl + i] = xbuf[t * a.xs + x0 + n - a.cpl + i];
     }
-    if (tid == 0) {
+    if (!LEAN && tid == 0) {
       for (int p = 0; p < L; ++p) {
         const int d = dly[p];
         if (d >= 0 && d < a.keep_below) csirow[d] = cadd(csirow[d], abar[p * a.words + w]);
@@ -416,12 +420,22 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     }
     __syncthreads();
     // receive FFT of every antenna and, in the same passes, the CSI row's FFT over the taps
-    lds_fft<-1, FFTV>(xbuf + x0, kRows + 1, a.xs, a.log2n, tw);
+    lds_fft<-1, FFTV>(xbuf + x0, kRows + (LEAN ? 0 : 1), a.xs, a.log2n, tw);
 
     // ideal CSI, detection, demapping and bit errors per subcarrier
     for (int q = tid; q < n; q += nthreads) {
       const int bin = subcarrier_bin(q, n, a.dc);
-      const float2 c = csirow[bin];
+      float2 c;
+      if (LEAN) {
+        // the same DFT over the taps, path by path: c = sum_p abar_p W^(bin d_p)
+        c = make_float2(0.f, 0.f);
+        for (int p = 0; p < L; ++p) {
+          const int d = dly[p];
+          if (d >= 0 && d < a.keep_below) c = cfma(abar[p * a.words + w], tw[(bin * d) & (n - 1)], c);
+        }
+      } else {
+        c = csirow[bin];
+      }
       // opaque zero offset: keeps the detector matrices as per-iteration LDS broadcast reads
       // instead of ~70 registers hoisted for the whole kernel
       int dofs = 0;
@@ -496,10 +510,10 @@ batch_kernel(BatchArgs a) {
 
 // Heterogeneous batches: workgroup i simulates the drop plans[i] describes; its first_drop is
 // the drop's Philox index and its per-drop pointers already point at the drop's entries.
-template <int NT, int NR, int PAIRS, int MAXT>
+template <int NT, int NR, int PAIRS, int MAXT, bool LEAN = false>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu<NT, MAXT>())))
 batch_kernel_multi(const BatchArgs* __restrict__ plans) {
-  drop_body<NT, NR, PAIRS, fft_values<NT, MAXT>()>(plans[blockIdx.x], 0);
+  drop_body<NT, NR, PAIRS, fft_values<NT, MAXT>(), LEAN>(plans[blockIdx.x], 0);
 }
 
 int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
@@ -507,11 +521,52 @@ int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
 struct Plan {
   int threads = 0, pairs = 0;
   int64_t lds = 0;
+  bool lean = false;      // Taylor table and twiddles in device memory, ideal CSI per subcarrier
+  int64_t ws_elems = 0;   // complex64 workspace elements of a lean drop
   BatchArgs args{};
 };
 
+constexpr int64_t kMaxLds = 160 * 1024;
+
+// LDS layout of a plan; a lean layout leaves out the Taylor table, the twiddles and the CSI row.
+int64_t layout_lds(BatchArgs& a, int nr, int nt, bool lean) {
+  const int rows = std::max(nr, nt) + (lean ? 0 : 1);  // signal rows (+ the CSI row)
+  int64_t o = 0;
+  a.off_x = 0;
+  const int64_t xbytes = static_cast<int64_t>(rows) * a.xs * sizeof(float2);
+  const int64_t uni_bytes = align16(static_cast<int64_t>(a.nnorm) * sizeof(double));
+  const int64_t sos_bytes = align16(static_cast<int64_t>(a.paths) * a.terms * 3 * sizeof(double));
+  // S (float64), then the detector's outputs and Jacobi workspace
+  const int64_t s64_bytes =
+      static_cast<int64_t>(2 * nr * nt + nt + 2 * nt * nt + 2 * nt * nr + kEigWork) * sizeof(double);
+  o += align16(std::max(xbytes, uni_bytes + sos_bytes + s64_bytes));
+  a.off_tmp_sos = static_cast<int>(uni_bytes);
+  a.off_tmp_s64 = static_cast<int>(uni_bytes + sos_bytes);
+  a.off_coef = static_cast<int>(o);
+  if (!lean) o += align16(static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs * sizeof(float2));
+  a.off_abar = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.paths) * a.words * sizeof(float2));
+  a.off_tw = static_cast<int>(o);
+  if (!lean) o += align16(static_cast<int64_t>(a.n) * sizeof(float2));
+  a.off_bits = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(nt) * a.bits_words * sizeof(uint32_t));
+  a.off_cons = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.order) * sizeof(float2));
+  a.off_s = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(nr) * nt * sizeof(float2));
+  a.off_det = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(nt * nt + nt * nr) * sizeof(float2) + nt * sizeof(float));
+  a.off_red = static_cast<int>(o);
+  o += 16;
+  a.off_dly = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.paths) * sizeof(int32_t));
+  return o;
+}
+
 // Validates the link (and frame, if any) and computes the launch geometry and LDS layout.
-int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out) {
+// allow_lean: an 8x8 link beyond one CU's LDS gets the lean layout (heterogeneous batches,
+// which provide its workspace).
+int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out, bool allow_lean = false) {
   HPX_REQUIRE(l, "hpx_run_batch: null link");
   if (f) {
     HPX_REQUIRE(f->elements && f->word_data, "hpx_run_batch_frames: null frame arrays");
@@ -614,38 +669,14 @@ int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out) {
   out->pairs = pairs;
 
   // LDS layout; the prologue tables (uniforms, SoS table) alias the signal buffer
-  const int rows = std::max(nr, nt) + 1;  // signal rows + the CSI row
-  int64_t o = 0;
-  a.off_x = 0;
-  const int64_t xbytes = static_cast<int64_t>(rows) * a.xs * sizeof(float2);
-  const int64_t uni_bytes = align16(static_cast<int64_t>(a.nnorm) * sizeof(double));
-  const int64_t sos_bytes = align16(static_cast<int64_t>(a.paths) * a.terms * 3 * sizeof(double));
-  // S (float64), then the detector's outputs and Jacobi workspace
-  const int64_t s64_bytes =
-      static_cast<int64_t>(2 * nr * nt + nt + 2 * nt * nt + 2 * nt * nr + kEigWork) * sizeof(double);
-  o += align16(std::max(xbytes, uni_bytes + sos_bytes + s64_bytes));
-  a.off_tmp_sos = static_cast<int>(uni_bytes);
-  a.off_tmp_s64 = static_cast<int>(uni_bytes + sos_bytes);
-  a.off_coef = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs * sizeof(float2));
-  a.off_abar = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(a.paths) * a.words * sizeof(float2));
-  a.off_tw = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(a.n) * sizeof(float2));
-  a.off_bits = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(nt) * a.bits_words * sizeof(uint32_t));
-  a.off_cons = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(a.order) * sizeof(float2));
-  a.off_s = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(nr) * nt * sizeof(float2));
-  a.off_det = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(nt * nt + nt * nr) * sizeof(float2) + nt * sizeof(float));
-  a.off_red = static_cast<int>(o);
-  o += 16;
-  a.off_dly = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(a.paths) * sizeof(int32_t));
+  int64_t o = layout_lds(a, nr, nt, false);
+  if (o > kMaxLds && allow_lean && nt == 8) {
+    o = layout_lds(a, nr, nt, true);
+    out->lean = true;
+    out->ws_elems = static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs + a.n;
+  }
   out->lds = o;
-  if (o > 160 * 1024) {
+  if (o > kMaxLds) {
     set_error("hpx_run_batch: link needs %lld bytes of LDS per drop (max 163840)", static_cast<long long>(o));
     return HPX_ERR_UNSUPPORTED;
   }
@@ -688,13 +719,13 @@ int launch_pairs(const Plan& plan, int64_t num_drops, hipStream_t stream) {
   return HPX_ERR_UNSUPPORTED;
 }
 
-template <int NT, int MAXT>
-int launch_multi(const BatchArgs* plans, const hpx_uneven_class& c, hipStream_t stream) {
+template <int NT, int MAXT, bool LEAN>
+int launch_multi_kernel(const BatchArgs* plans, const hpx_uneven_class& c, hipStream_t stream) {
   const void* fn = nullptr;
   switch (c.pairs) {
-    case 1: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 1, MAXT>); break;
-    case 2: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 2, MAXT>); break;
-    case 4: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 4, MAXT>); break;
+    case 1: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 1, MAXT, LEAN>); break;
+    case 2: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 2, MAXT, LEAN>); break;
+    case 4: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 4, MAXT, LEAN>); break;
     default: set_error("hpx_uneven_run: unsupported samples per thread"); return HPX_ERR_UNSUPPORTED;
   }
   if (!ensure_lds(fn, c.lds_bytes)) {
@@ -703,14 +734,37 @@ int launch_multi(const BatchArgs* plans, const hpx_uneven_class& c, hipStream_t 
   }
   const BatchArgs* first = plans + c.first;
   switch (c.pairs) {
-    case 1: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 1, MAXT>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
-    case 2: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 2, MAXT>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
-    default: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 4, MAXT>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
+    case 1: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 1, MAXT, LEAN>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
+    case 2: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 2, MAXT, LEAN>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
+    default: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 4, MAXT, LEAN>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
   }
   return check_launch("hpx_uneven_run");
 }
 
+// lean classes exist only for 8x8 links (the one shape beyond one CU's LDS)
+template <int NT, int MAXT>
+int launch_multi(const BatchArgs* plans, const hpx_uneven_class& c, hipStream_t stream) {
+  if constexpr (NT == 8) {
+    if (c.lean) return launch_multi_kernel<NT, MAXT, true>(plans, c, stream);
+  }
+  if (c.lean) {
+    set_error("hpx_uneven_run: lean class with %d antennas", NT);
+    return HPX_ERR_INVALID;
+  }
+  return launch_multi_kernel<NT, MAXT, false>(plans, c, stream);
+}
+
 }  // namespace
+
+extern "C" int64_t hpx_uneven_workspace_elements(const hpx_link* link_host) {
+  if (!link_host) {
+    set_error("hpx_uneven_workspace_elements: null pointer");
+    return -1;
+  }
+  Plan plan;
+  if (make_plan(link_host, nullptr, &plan, true) != HPX_OK) return -1;
+  return plan.lean ? plan.ws_elems : 0;
+}
 
 extern "C" int64_t hpx_uneven_plan_bytes(void) { return static_cast<int64_t>(sizeof(BatchArgs)); }
 
@@ -732,9 +786,15 @@ extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tab
                 static_cast<long long>(i));
     HPX_REQUIRE((tb.corr_rx == nullptr) == (tb.corr_tx == nullptr),
                 "hpx_uneven_prepare: give both correlation matrices or none");
-    const int st = make_plan(&links_host[i], nullptr, &plans[i]);
+    const int st = make_plan(&links_host[i], nullptr, &plans[i], true);
     if (st != HPX_OK) return st;
     BatchArgs& a = plans[i].args;
+    if (plans[i].lean) {
+      HPX_REQUIRE(tb.workspace, "hpx_uneven_prepare: drop %lld needs a workspace of %lld elements",
+                  static_cast<long long>(i), static_cast<long long>(plans[i].ws_elems));
+      a.ws_coef = reinterpret_cast<float2*>(tb.workspace);
+      a.ws_tw = reinterpret_cast<float2*>(tb.workspace) + (plans[i].ws_elems - a.n);
+    }
     a.delays = tb.delays;
     a.path_gains = tb.path_gains;
     a.corr_rx = reinterpret_cast<const double*>(tb.corr_rx);
@@ -744,9 +804,9 @@ extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tab
     a.seed = seed;
     a.first_drop = drop_index_host[i];
   }
-  // kernel class = (antennas, threads, samples per thread); heavier classes and drops first
+  // kernel class = (antennas, threads, samples per thread, lean); heavier classes and drops first
   auto key = [&](int64_t i) {
-    return std::make_tuple(links_host[i].num_tx, plans[i].threads, plans[i].pairs);
+    return std::make_tuple(links_host[i].num_tx, plans[i].threads, plans[i].pairs, plans[i].lean ? 1 : 0);
   };
   auto cost = [&](int64_t i) {
     const hpx_link& l = links_host[i];
@@ -754,7 +814,7 @@ extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tab
   };
   std::vector<int64_t> idx(static_cast<size_t>(num_drops));
   for (int64_t i = 0; i < num_drops; ++i) idx[i] = i;
-  std::map<std::tuple<int, int, int>, double> class_cost;
+  std::map<std::tuple<int, int, int, int>, double> class_cost;
   for (int64_t i = 0; i < num_drops; ++i) class_cost[key(i)] = std::max(class_cost[key(i)], cost(i));
   std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
     const auto kx = key(x), ky = key(y);
@@ -782,7 +842,7 @@ extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tab
       c.num_tx = links_host[i].num_tx;
       c.threads = plans[i].threads;
       c.pairs = plans[i].pairs;
-      c.reserved = 0;
+      c.lean = plans[i].lean ? 1 : 0;
       c.lds_bytes = 0;
     }
     hpx_uneven_class& c = classes_host[nc - 1];

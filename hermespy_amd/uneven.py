@@ -9,7 +9,10 @@ population into kernel classes (antennas, threads per drop, samples per
 thread), writes one plan per drop (``hpx_uneven_prepare``) and replays one
 launch per class from a captured graph (``hpx_uneven_run``); drop i's result
 depends only on (seed, its drop index, its link), exactly as in
-``hpx_run_batch``.  A per-class autotuner picks the workgroup size.
+``hpx_run_batch``.  A per-class autotuner picks the workgroup size.  8x8 drops
+at 2048 subcarriers, whose working set exceeds one CU's LDS, run in lean
+classes: their Taylor table and twiddles live in a per-drop device workspace
+and the ideal CSI is summed path by path per subcarrier.
 """
 
 from __future__ import annotations
@@ -49,11 +52,11 @@ class DropShape:
                           2 * np.pi * self.doppler_hz, detector="mmse", name="uneven")
 
 
-def draw_population(num_drops: int, seed: int = 20261202, exclude=((8, 2048),)) -> list:
+def draw_population(num_drops: int, seed: int = 20261202, exclude=()) -> list:
     """The mixed population: antennas {1, 2, 4, 8} square, N in {128, ..., 2048} with CP N/8, 5..23
     sample-spaced paths with an exponential profile of 1..6 dB per tap, Doppler 10..900 Hz,
     QPSK / 16-QAM / 64-QAM, SNR from 0..30 dB in 3 dB steps.  Shapes in `exclude` (antennas, N)
-    are redrawn: a 8x8 drop at 2048 subcarriers needs more than one CU's 160 KB of LDS."""
+    are redrawn."""
 
     rng = np.random.default_rng(seed)
     out = []
@@ -101,6 +104,15 @@ class UnevenBatch:
         for i in range(n):
             self.tables[i].delays = self.delays.data_ptr() + 4 * int(offs[i])
             self.tables[i].path_gains = self.gains.data_ptr() + 8 * 2 * int(offs[i])
+        # drops beyond one CU's LDS (8x8 at 2048 subcarriers) keep their Taylor and twiddle
+        # tables in device workspaces
+        ws = np.array([_lib.check_count(self.lib.hpx_uneven_workspace_elements(ctypes.byref(_c_link(c))))
+                       for c in self.links], dtype=np.int64)
+        self.workspace = torch.empty(max(1, int(ws.sum())), dtype=torch.complex64, device=dev)
+        ws_offs = np.concatenate([[0], np.cumsum(ws)[:-1]])
+        for i in np.flatnonzero(ws):
+            self.tables[i].workspace = self.workspace.data_ptr() + 8 * int(ws_offs[i])
+        self.num_lean = int(np.count_nonzero(ws))
         self.noise_power = torch.as_tensor([c.noise_power(s.snr_db) for c, s in zip(self.links, shapes)],
                                            dtype=torch.float64, device=dev)
         self.errors = torch.zeros(n, dtype=torch.int32, device=dev)
@@ -182,7 +194,7 @@ class UnevenBatch:
             c = self.classes[k]
             i = int(self.order[c.first])
             out.append({"antennas": c.num_tx, "num_subcarriers": self.links[i].num_subcarriers, "drops": c.count,
-                        "threads": c.threads, "pairs": c.pairs, "lds_bytes": c.lds_bytes})
+                        "threads": c.threads, "pairs": c.pairs, "lean": bool(c.lean), "lds_bytes": c.lds_bytes})
         return out
 
     def ber_by_snr(self) -> dict:

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define HPX_ABI_VERSION 4
+#define HPX_ABI_VERSION 5
 
 typedef enum hpx_status {
   HPX_OK = 0,
@@ -418,13 +418,22 @@ typedef struct hpx_drop_tables {
   const double* path_gains;
   const hpx_c128* corr_rx;   /* or NULL */
   const hpx_c128* corr_tx;
+  hpx_c64* workspace;        /* hpx_uneven_workspace_elements(link) elements of device memory for a
+                                drop whose link exceeds the LDS of one CU, else NULL */
 } hpx_drop_tables;
 
 typedef struct hpx_uneven_class {
   int64_t first, count;      /* contiguous plan slots of the class */
-  int32_t num_tx, threads, pairs, reserved;
+  int32_t num_tx, threads, pairs, lean; /* lean: drops that keep the Taylor table and the
+                                            twiddles in their workspace and compute the ideal
+                                            CSI per subcarrier (8x8 at 2048 subcarriers) */
   int64_t lds_bytes;         /* dynamic LDS per workgroup (max over the class) */
 } hpx_uneven_class;
+
+/* Complex64 elements of device workspace a drop of `link` needs in a heterogeneous batch: 0 when
+ * its working set fits one CU's LDS, the Taylor table plus the twiddle table when it fits only
+ * without them (and without the ideal-CSI row), -1 (with hpx_last_error) when unsupported. */
+int64_t hpx_uneven_workspace_elements(const hpx_link* link_host);
 
 /* Bytes of one drop plan. */
 int64_t hpx_uneven_plan_bytes(void);

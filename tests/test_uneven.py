@@ -15,7 +15,8 @@ def test_population_covers_the_mix():
     ants = {s.antennas for s in pop}
     ns = {s.num_subcarriers for s in pop}
     assert ants == {1, 2, 4, 8} and ns == {128, 256, 512, 1024, 2048}
-    assert not any(s.antennas == 8 and s.num_subcarriers == 2048 for s in pop)
+    assert any(s.antennas == 8 and s.num_subcarriers == 2048 for s in pop)
+    assert not any(s.antennas == 8 and s.num_subcarriers == 2048 for s in draw_population(500, 3, ((8, 2048),)))
     assert {s.modulation_order for s in pop} == {4, 16, 64}
     paths = np.array([s.num_paths for s in pop])
     assert paths.min() == 5 and paths.max() == 23
@@ -24,8 +25,9 @@ def test_population_covers_the_mix():
 
 
 def test_planner_groups_by_kernel_class(native_lib):
-    """hpx_uneven_prepare on the host: classes are contiguous runs of one (antennas, threads, pairs),
-    heavier classes first, every drop placed once, LDS = max over the class."""
+    """hpx_uneven_prepare on the host: classes are contiguous runs of one (antennas, threads, pairs,
+    lean), heavier classes first, every drop placed once, LDS = max over the class; 8x8 drops at
+    2048 subcarriers form lean classes within one CU's LDS."""
 
     from hermespy_amd import _lib
     from hermespy_amd.uneven import MAX_CLASSES, _c_link, draw_population
@@ -35,9 +37,11 @@ def test_planner_groups_by_kernel_class(native_lib):
     n = len(links)
     c_links = (_lib.HpxLink * n)(*[_c_link(c) for c in links])
     tables = (_lib.HpxDropTables * n)()
+    ws = [int(native_lib.hpx_uneven_workspace_elements(ctypes.byref(c_links[i]))) for i in range(n)]
     for i in range(n):
         tables[i].delays = 0x1000 + 256 * i  # host planning never dereferences device pointers
         tables[i].path_gains = 0x100000 + 512 * i
+        tables[i].workspace = 0x10000000 + 0x1000000 * i if ws[i] else None
     idx = np.arange(n, dtype=np.uint32) + 17
     plan_bytes = int(native_lib.hpx_uneven_plan_bytes())
     plans = np.zeros(n * plan_bytes, dtype=np.uint8)
@@ -48,18 +52,59 @@ def test_planner_groups_by_kernel_class(native_lib):
                                              plans.ctypes.data, order.ctypes.data, classes, MAX_CLASSES,
                                              ctypes.byref(nc)))
     assert sorted(order.tolist()) == list(range(n))
-    seen = 0
+    seen = lean_classes = 0
     for k in range(nc.value):
         c = classes[k]
         assert c.first == seen and c.count > 0
         members = order[c.first:c.first + c.count]
         shapes = {(links[i].num_tx, links[i].num_subcarriers) for i in members}
         assert len(shapes) == 1 and next(iter(shapes))[0] == c.num_tx
-        lds = [int(native_lib.hpx_run_batch_lds_bytes(ctypes.byref(c_links[i]))) for i in members]
-        assert c.lds_bytes == max(lds)
+        lean = next(iter(shapes)) == (8, 2048)
+        assert c.lean == int(lean) and all((ws[i] > 0) == lean for i in members)
+        if lean:
+            lean_classes += 1
+            assert 0 < c.lds_bytes <= 160 * 1024
+        else:
+            lds = [int(native_lib.hpx_run_batch_lds_bytes(ctypes.byref(c_links[i]))) for i in members]
+            assert c.lds_bytes == max(lds)
         assert c.threads * c.pairs * 2 == links[members[0]].num_subcarriers
         seen += c.count
-    assert seen == n
+    assert seen == n and lean_classes >= 1
+
+
+def test_lean_drops_need_a_workspace(native_lib):
+    """An 8x8 link at 2048 subcarriers exceeds one CU's LDS: the single-link path refuses it, the
+    heterogeneous planner sizes its workspace (Taylor table of every path and block, then N
+    twiddles) and refuses the drop without one."""
+
+    from hermespy_amd import _lib
+    from hermespy_amd.uneven import MAX_CLASSES, DropShape, _c_link
+
+    big = _c_link(DropShape(8, 2048, 23, 2.0, 100.0, 16, 10.0).link())
+    assert native_lib.hpx_run_batch_lds_bytes(ctypes.byref(big)) == 0
+    ws = native_lib.hpx_uneven_workspace_elements(ctypes.byref(big))
+    assert ws > 2048 and (ws - 2048) % (23 * 5) == 0
+    fewer = _c_link(DropShape(8, 2048, 5, 2.0, 100.0, 16, 10.0).link())
+    assert (native_lib.hpx_uneven_workspace_elements(ctypes.byref(fewer)) - 2048) * 23 == (ws - 2048) * 5
+    for shape in ((8, 1024), (4, 2048), (1, 128)):
+        small = _c_link(DropShape(shape[0], shape[1], 23, 2.0, 100.0, 16, 10.0).link())
+        assert native_lib.hpx_uneven_workspace_elements(ctypes.byref(small)) == 0
+    bad = _c_link(DropShape(3, 256, 5, 2.0, 100.0, 16, 10.0).link())
+    assert native_lib.hpx_uneven_workspace_elements(ctypes.byref(bad)) == -1
+    with pytest.raises(ValueError, match="not instantiated"):
+        _lib.check_count(-1)
+    c_links = (_lib.HpxLink * 1)(big)
+    tables = (_lib.HpxDropTables * 1)()
+    tables[0].delays, tables[0].path_gains = 0x1000, 0x2000
+    idx = np.zeros(1, dtype=np.uint32)
+    plans = np.zeros(int(native_lib.hpx_uneven_plan_bytes()), dtype=np.uint8)
+    order = np.empty(1, dtype=np.int64)
+    classes = (_lib.HpxUnevenClass * MAX_CLASSES)()
+    nc = _lib.c_int(0)
+    st = native_lib.hpx_uneven_prepare(c_links, tables, idx.ctypes.data, 1, ctypes.c_uint64(0), 0x10, 0x20,
+                                       plans.ctypes.data, order.ctypes.data, classes, MAX_CLASSES, ctypes.byref(nc))
+    with pytest.raises(ValueError, match="needs a workspace"):
+        _lib.check(st)
 
 
 def test_planner_rejects_bad_links(native_lib):
@@ -102,6 +147,8 @@ def test_uneven_graph_replay_matches_single_link_launches(cuda):
         assert np.array_equal(ub.errors.cpu().numpy(), eager), rep
     # each drop equals the plain single-link launch of the same drop
     for i in range(0, len(pop), 7):
+        if (pop[i].antennas, pop[i].num_subcarriers) == (8, 2048):
+            continue  # beyond the single-link path's LDS: checked against the oracle below
         cfg = pop[i].link()
         bl = BatchedLink(cfg)
         npw = torch.as_tensor([cfg.noise_power(pop[i].snr_db)], dtype=torch.float64, device=cuda)
@@ -133,3 +180,39 @@ def test_uneven_autotuner_keeps_results(cuda):
     torch.cuda.synchronize()
     assert np.array_equal(ub.errors.cpu().numpy(), before)
     assert info["choice"]
+
+
+@pytest.mark.gpu
+def test_lean_drops_match_the_oracle(cuda):
+    """8x8 drops at 2048 subcarriers (Taylor table and twiddles in device workspace, ideal CSI
+    summed path by path) in a mixed batch: each drop's error count agrees with the float64
+    oracle's up to symbols within rounding of a decision boundary, eagerly and on graph replay."""
+
+    import torch
+
+    from conftest import oracle_link
+    from hermespy_amd.uneven import DropShape, UnevenBatch
+    from oracle import drop as odrop
+
+    lean = [DropShape(8, 2048, 23, 1.5, 400.0, 16, 12.0), DropShape(8, 2048, 5, 4.0, 60.0, 4, 6.0),
+            DropShape(8, 2048, 14, 2.5, 850.0, 64, 24.0), DropShape(8, 2048, 9, 1.0, 200.0, 16, 30.0)]
+    others = [DropShape(8, 1024, 11, 2.0, 300.0, 16, 15.0), DropShape(2, 2048, 7, 3.0, 100.0, 64, 21.0)]
+    pop = lean + others
+    print("lean 8x8 @ 2048 drops vs oracle", flush=True)
+    ub = UnevenBatch(pop, seed=77, first_drop=500)
+    assert ub.num_lean == len(lean)
+    ub.run()
+    torch.cuda.synchronize()
+    eager = ub.errors.cpu().numpy().copy()
+    ub.capture()
+    for rep in range(2):
+        ub.errors.fill_(-1)
+        torch.cuda.synchronize()
+        ub.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(ub.errors.cpu().numpy(), eager), rep
+    for i, shape in enumerate(pop):
+        cfg = shape.link()
+        ref = odrop.run_drop(oracle_link(cfg), 77, 500 + i, shape.snr_db, faithful=False)
+        assert ref.errors > 0
+        assert abs(int(eager[i]) - ref.errors) <= max(2, int(2e-4 * ref.bits)), (i, shape, int(eager[i]), ref.errors)
