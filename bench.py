@@ -46,9 +46,11 @@ def algorithmic_flops_per_drop(cfg) -> float:
 
 
 def _oracle_drops(args) -> int:
-    """Worker: reference-faithful oracle drops for budget_s seconds on one core; returns the count."""
+    """Worker: oracle drops for budget_s seconds on one core (reference-faithful unless faithful is
+    False: closed-form CSI and vectorised taps); returns the count."""
 
-    cfg_name, first, stride, budget_s = args
+    cfg_name, first, stride, budget_s = args[:4]
+    faithful = args[4] if len(args) > 4 else True
     from threadpoolctl import threadpool_limits
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -65,14 +67,16 @@ def _oracle_drops(args) -> int:
         t0 = time.perf_counter()
         while time.perf_counter() - t0 < budget_s or done == 0:
             d = first + done * stride
-            odrop.run_drop(link, SEED, d, float(snr[d % snr.size]), faithful=True)
+            odrop.run_drop(link, SEED, d, float(snr[d % snr.size]), faithful=faithful)
             done += 1
     return done
 
 
-def cpu_baseline(cfg, budget_s: float = 15.0, all_core_budget_s: float = 10.0, workers: int = 16) -> dict:
+def cpu_baseline(cfg, budget_s: float = 15.0, all_core_budget_s: float = 10.0, workers: int = 16,
+                 vectorised_budget_s: float = 5.0) -> dict:
     """Reference-faithful float64 oracle (dense impulse response, per-path loop, pinv/solve): one host core,
-    then one process per core for the all-core aggregate (as the reference runs one actor per CPU)."""
+    then one process per core for the all-core aggregate (as the reference runs one actor per CPU), and
+    the oracle's vectorised mode (closed-form CSI, no dense impulse response) on one core for fairness."""
 
     from concurrent.futures import ProcessPoolExecutor
 
@@ -90,6 +94,12 @@ def cpu_baseline(cfg, budget_s: float = 15.0, all_core_budget_s: float = 10.0, w
             counts = list(pool.map(_oracle_drops, [(name, w, workers, all_core_budget_s) for w in range(workers)]))
         dt = time.perf_counter() - t0
         out["all_cores"] = {"value": sum(counts) / dt, "cores": workers, "drops": sum(counts), "seconds": dt}
+    if vectorised_budget_s > 0:
+        t0 = time.perf_counter()
+        done = _oracle_drops((name, 0, 1, vectorised_budget_s, False))
+        dt = time.perf_counter() - t0
+        out["vectorised"] = {"value": done / dt, "cores": 1, "drops": done, "seconds": dt,
+                             "mode": "closed-form CSI, vectorised taps, float64"}
     return out
 
 

@@ -85,7 +85,13 @@ struct BatchArgs {
   // lean drops (working set beyond one CU's LDS): Taylor table and twiddles in device memory
   float2* ws_coef;
   float2* ws_tw;
+  int off_tmp_dir;  // prologue only: per-term word-mean weights
 };
+
+__device__ __forceinline__ bool path_in_range(const BatchArgs& a, int p) {
+  const int d = a.delays[p];
+  return d >= 0 && d <= a.maxd;
+}
 
 // bps bits starting at bit `bit` of an LSB-first word window (bit i of the result = i-th bit)
 __device__ __forceinline__ uint32_t symbol_bits(const uint32_t* bw, int bit, int bps) {
@@ -115,6 +121,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
   double* uni = reinterpret_cast<double*>(lds + a.off_x);        // prologue only
   double* sos = reinterpret_cast<double*>(lds + a.off_tmp_sos);  // prologue only
   double* s64 = reinterpret_cast<double*>(lds + a.off_tmp_s64);  // prologue only: S in float64
+  double* amp_dir = reinterpret_cast<double*>(lds + a.off_tmp_dir);  // prologue only: [L][terms]
   float2* det_v = reinterpret_cast<float2*>(lds + a.off_det);     // [NT][NT] eigenvectors of S^H S
   float2* det_a = det_v + NT * NT;                                 // [NT][NR] V^H S^H
   float* det_l = reinterpret_cast<float*>(det_a + NT * NR);        // [NT] eigenvalues
@@ -139,7 +146,9 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     }
   }
   if (tid == 0) red[0] = 0;
-  for (int p = tid; p < L; p += nthreads) dly[p] = a.delays[p];
+  // a path whose delay lies outside [0, max delay] contributes nothing: it gets delay 0 and zero
+  // taps (Taylor coefficients and word means), so the channel loop needs no per-path check
+  for (int p = tid; p < L; p += nthreads) dly[p] = path_in_range(a, p) ? a.delays[p] : 0;
   __syncthreads();
 
   // spatial response S = R_rx exp(2 pi j U)[:NR, :NT] R_tx (fading.py:476-489)
@@ -186,6 +195,8 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
         row[1] = -M_PI + 2.0 * M_PI * uni[o_los_ph + p];
         row[2] = a.path_gains[p];
       }
+      // the word means' Dirichlet weight depends on the term only, not on the word
+      amp_dir[e] = row[2] * dirichlet_factor(row[0], n);
     }
   }
   // constellation indexed by transmitted bit order (scaled by sqrt(P_tx / N)); twiddles
@@ -227,12 +238,20 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
   if (tid < taylor_threads) {
     for (int e = tid; e < L * a.nb; e += taylor_threads) {
       const int p = e / a.nb, b = e - p * a.nb;
-      taylor_coeffs(sos + 3 * p * a.terms, a.terms, tb.centre(b), tb.half_width(), coef + e * kTaylorCoeffs);
+      if (path_in_range(a, p)) {
+        taylor_coeffs(sos + 3 * p * a.terms, a.terms, tb.centre(b), tb.half_width(), coef + e * kTaylorCoeffs);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kTaylorCoeffs; ++k) coef[e * kTaylorCoeffs + k] = make_float2(0.f, 0.f);
+      }
     }
   }
   for (int e = tid; e < L * a.words; e += nthreads) {
     const int p = e / a.words, w = e - p * a.words;
-    const double2 m = tap_mean(sos + 3 * p * a.terms, a.terms, static_cast<double>(w) * (n + a.cp) + a.cp, n);
+    const double2 m = path_in_range(a, p)
+                          ? tap_mean_weighted(sos + 3 * p * a.terms, amp_dir + p * a.terms, a.terms,
+                                              static_cast<double>(w) * (n + a.cp) + a.cp, n)
+                          : make_double2(0.0, 0.0);
     abar[e] = make_float2(static_cast<float>(m.x), static_cast<float>(m.y));
   }
   __syncthreads();
@@ -328,10 +347,10 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
           // no delayed index reaches before the frame start: no per-lane checks
           for (int p = 0; p < L; ++p) {
             const int d = dly[p];
-            if (d < 0 || d > a.maxd) continue;
             const int m = static_cast<int>(nabs) - d;
             const int b0 = m >> a.log2b;
-            const float2* c0 = coef + (p * a.nb + b0) * kTaylorCoeffs;
+            // (24-bit multiply: full rate, where a 32-bit one takes four passes)
+            const float2* c0 = coef + __umul24(static_cast<unsigned>(p * a.nb + b0), kTaylorCoeffs);
             const float s0 = tb.offset(m);
             const float2 h0 = taylor_eval(c0, s0);
             // the pair's second sample opens the next block when m ends one
@@ -347,7 +366,6 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
         } else {
           for (int p = 0; p < L; ++p) {
             const int d = dly[p];
-            if (d < 0 || d > a.maxd) continue;
             const int64_t m = nabs - d;
             const float2* xc = xbuf + x0 + nl - d;
             if (m >= 0) {
@@ -539,9 +557,12 @@ int64_t layout_lds(BatchArgs& a, int nr, int nt, bool lean) {
   // S (float64), then the detector's outputs and Jacobi workspace
   const int64_t s64_bytes =
       static_cast<int64_t>(2 * nr * nt + nt + 2 * nt * nt + 2 * nt * nr + kEigWork) * sizeof(double);
-  o += align16(std::max(xbytes, uni_bytes + sos_bytes + s64_bytes));
+  const int64_t s64_aligned = align16(s64_bytes);
+  const int64_t dir_bytes = align16(static_cast<int64_t>(a.paths) * a.terms * sizeof(double));
+  o += align16(std::max(xbytes, uni_bytes + sos_bytes + s64_aligned + dir_bytes));
   a.off_tmp_sos = static_cast<int>(uni_bytes);
   a.off_tmp_s64 = static_cast<int>(uni_bytes + sos_bytes);
+  a.off_tmp_dir = static_cast<int>(uni_bytes + sos_bytes + s64_aligned);
   a.off_coef = static_cast<int>(o);
   if (!lean) o += align16(static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs * sizeof(float2));
   a.off_abar = static_cast<int>(o);

@@ -85,13 +85,21 @@ struct TaylorBlocks {
   }
   __device__ __forceinline__ double half_width() const { return 0.5 * static_cast<double>(1 << log2b); }
   // normalised offset (2 r - (B - 1)) / B of sample m in its block (r = m mod B), exact in
-  // float32; B is a power of two, so the division is an exact scaling by 2^-log2b
+  // float32: r 2^(1-log2b) and 2^-log2b - 1 are exact, and so is their sum (an odd integer
+  // of magnitude < 2^12 over B), which the single rounding of the FMA therefore keeps
   __device__ __forceinline__ float offset(int64_t m) const {
     const int r = static_cast<int>(m & ((int64_t(1) << log2b) - 1));  // 0 .. B-1
-    const float B = static_cast<float>(1 << log2b);
-    return ldexpf(2.0f * static_cast<float>(r) - (B - 1.0f), -log2b);
+    return fmaf(static_cast<float>(r), ldexpf(2.0f, -log2b), ldexpf(1.0f, -log2b) - 1.0f);
   }
 };
+
+// sin(n rate / 2) / (n sin(rate / 2)): the mean of exp(j rate k) over k = 0 .. n-1 relative to its
+// value at the window centre
+__device__ inline double dirichlet_factor(double rate, int n) {
+  const double half = 0.5 * rate;
+  const double den = static_cast<double>(n) * sin(half);
+  return (den != 0.0) ? sin(static_cast<double>(n) * half) / den : 1.0;
+}
 
 // Word-mean of one path over N samples starting at m0 (the Dirichlet closed form
 // of the per-symbol average of hermespy/simulation/modem/channel_estimation.py:242).
@@ -101,14 +109,29 @@ __device__ inline double2 tap_mean(const double* sos, int terms, double m0, int 
   for (int k = 0; k < terms; ++k) {
     const double rate = sos[3 * k], amp = sos[3 * k + 2];
     if (amp == 0.0) continue;
-    const double half = 0.5 * rate;
-    const double den = static_cast<double>(n) * sin(half);
-    const double dir = (den != 0.0) ? sin(static_cast<double>(n) * half) / den : 1.0;
+    const double dir = dirichlet_factor(rate, n);
     const double ph = reduce_phase(fma(rate, centre, sos[3 * k + 1]));
     double s, c;
     sincos(ph, &s, &c);
     ar += amp * dir * c;
     ai += amp * dir * s;
+  }
+  return make_double2(ar, ai);
+}
+
+// tap_mean with the per-term weights amp_dir[k] = amp * dirichlet_factor(rate, n) computed once
+// for every word (the same value, so the same result)
+__device__ inline double2 tap_mean_weighted(const double* sos, const double* amp_dir, int terms, double m0, int n) {
+  const double centre = m0 + 0.5 * static_cast<double>(n - 1);
+  double ar = 0.0, ai = 0.0;
+  for (int k = 0; k < terms; ++k) {
+    if (sos[3 * k + 2] == 0.0) continue;
+    const double w = amp_dir[k];
+    const double ph = reduce_phase(fma(sos[3 * k], centre, sos[3 * k + 1]));
+    double s, c;
+    sincos(ph, &s, &c);
+    ar += w * c;
+    ai += w * s;
   }
   return make_double2(ar, ai);
 }
