@@ -148,7 +148,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
   if (tid == 0) red[0] = 0;
   // a path whose delay lies outside [0, max delay] contributes nothing: it gets delay 0 and zero
   // taps (Taylor coefficients and word means), so the channel loop needs no per-path check
-  for (int p = tid; p < L; p += nthreads) dly[p] = path_in_range(a, p) ? a.delays[p] : 0;
+  for (int p = tid; p <= L; p += nthreads) dly[p] = p < L && path_in_range(a, p) ? a.delays[p] : 0;
   __syncthreads();
 
   // spatial response S = R_rx exp(2 pi j U)[:NR, :NT] R_tx (fading.py:476-489)
@@ -345,8 +345,11 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
         const float step = 2.0f / static_cast<float>(1 << a.log2b);  // offset of the next sample in a block
         if (win0 >= a.maxd) {
           // no delayed index reaches before the frame start: no per-lane checks
+          // (the next path's delay is read one iteration ahead, off the loop's critical path)
+          int d_next = dly[0];
           for (int p = 0; p < L; ++p) {
-            const int d = dly[p];
+            const int d = d_next;
+            d_next = dly[p + 1];
             const int m = static_cast<int>(nabs) - d;
             const int b0 = m >> a.log2b;
             // (24-bit multiply: full rate, where a 32-bit one takes four passes)
@@ -580,7 +583,7 @@ int64_t layout_lds(BatchArgs& a, int nr, int nt, bool lean) {
   a.off_red = static_cast<int>(o);
   o += 16;
   a.off_dly = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(a.paths) * sizeof(int32_t));
+  o += align16(static_cast<int64_t>(a.paths + 1) * sizeof(int32_t));  // + a pad entry for the prefetch
   return o;
 }
 
