@@ -24,8 +24,8 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3  # published MI355X FP32 peak (packed FP32 math)
 FP32_UNPACKED_PEAK_TFLOPS = 78.6  # one FP32 FMA per lane and cycle: the kernels are built without packed FP32
 # HBM bytes of one fused 4x4 drop, from rocprofv3 FETCH_SIZE / WRITE_SIZE over 45 056-drop launches
 # (FETCH x2.0 and WRITE x1.0, the factors a 256 MiB device copy reads in the same process):
-# (3170.4 x 2 + 59113.3) KiB / 45056; almost all of it is register-spill stores
-HBM_BYTES_PER_DROP = (3170.42 * 2.0 + 59113.34) * 1024.0 / 45056.0
+# (3199.7 x 2 + 109351.6) KiB / 45056; almost all of it is register-spill stores
+HBM_BYTES_PER_DROP = (3199.73 * 2.0 + 109351.63) * 1024.0 / 45056.0
 DROPS_PER_STEP = 11 * 4096  # one step: the campaign's 4096 drops at each of its 11 SNR points, one launch
 SEED = 20261202
 
