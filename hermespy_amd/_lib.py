@@ -131,6 +131,7 @@ SIGNATURES = {
     "hpx_count_errors": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
     "hpx_run_batch": (c_int, [ctypes.POINTER(HpxLink), c_vp, c_vp, c_vp, c_vp, c_u64, c_i64, c_i64, c_vp, c_vp, c_vp,
                               c_vp]),
+    "hpx_run_batch_f64": (c_int, [ctypes.POINTER(HpxLink), c_vp, c_vp, c_u64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "hpx_run_batch_frames": (c_int, [ctypes.POINTER(HpxLink), ctypes.POINTER(HpxFrame), c_vp, c_vp, c_vp, c_vp,
                                      c_vp, c_u64, c_i64, c_i64, c_vp, ctypes.POINTER(HpxBatchOut), c_vp]),
     "hpx_uneven_plan_bytes": (c_i64, []),

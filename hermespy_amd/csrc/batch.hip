@@ -976,3 +976,249 @@ extern "C" int hpx_run_batch_drops(const hpx_link* link_host, const hpx_frame* f
   set_error("hpx_run_batch: unreachable antenna configuration");
   return HPX_ERR_UNSUPPORTED;
 }
+
+// ---------------------------------------------------------------- float64 single-antenna drops
+// The float64 counterpart of the fused drop for single-antenna links of up to 256 subcarriers:
+// the same counter-based streams and the same steps, every value a double, the taps evaluated
+// exactly (sum of sinusoids per sample, fading.py:293-343) instead of from Taylor blocks, and
+// the OFDM transforms as direct DFTs over a twiddle table.  One 256-thread workgroup per drop.
+namespace {
+
+constexpr int kF64Threads = 256;
+constexpr int kF64MaxSubcarriers = 256;
+
+struct F64Layout {
+  int off_uni, off_sos, off_tw, off_sym, off_x, off_y, off_abar, off_bits, off_red;
+  int dx, frame, bits_words;
+  int64_t bytes;
+};
+
+__global__ void __launch_bounds__(kF64Threads) f64_drop_kernel(BatchArgs a, F64Layout g, double2* equalized) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  double* uni = reinterpret_cast<double*>(lds + g.off_uni);
+  double* sos = reinterpret_cast<double*>(lds + g.off_sos);
+  double2* tw = reinterpret_cast<double2*>(lds + g.off_tw);    // [N] exp(-2 pi j k / N)
+  double2* sym = reinterpret_cast<double2*>(lds + g.off_sym);  // [N] one word's symbols at their bins
+  double2* x = reinterpret_cast<double2*>(lds + g.off_x);      // [dx + frame] transmitted, dx leading zeros
+  double2* y = reinterpret_cast<double2*>(lds + g.off_y);      // [frame] received
+  double2* abar = reinterpret_cast<double2*>(lds + g.off_abar);  // [L][words] word means of the taps
+  uint32_t* bits = reinterpret_cast<uint32_t*>(lds + g.off_bits);
+  int* red = reinterpret_cast<int*>(lds + g.off_red);
+  const int tid = threadIdx.x;
+  const int64_t local_drop = blockIdx.x;
+  const uint32_t drop = static_cast<uint32_t>(a.first_drop + local_drop);
+  const int n = a.n, L = a.paths, period = a.n + a.cp;
+
+  // realisation uniforms, the data bits (stream HPX_STREAM_BITS, LSB first), twiddles
+  for (int c = tid; 2 * c < a.nnorm; c += kF64Threads) {
+    const double2 z = normal_pair_f64(stream_call(a.seed, drop, HPX_STREAM_REALIZATION, static_cast<uint32_t>(c)));
+    uni[2 * c] = normal_cdf(z.x);
+    if (2 * c + 1 < a.nnorm) uni[2 * c + 1] = normal_cdf(z.y);
+  }
+  for (int c = tid; 4 * c < g.bits_words; c += kF64Threads) {
+    const uint4 r = stream_call(a.seed, drop, HPX_STREAM_BITS, static_cast<uint32_t>(c));
+    const uint32_t comp[4] = {r.x, r.y, r.z, r.w};
+    for (int k = 0; k < 4 && 4 * c + k < g.bits_words; ++k) bits[4 * c + k] = comp[k];
+  }
+  for (int k = tid; k < n; k += kF64Threads) {
+    double s, c;
+    sincospi(-2.0 * static_cast<double>(k) / static_cast<double>(n), &s, &c);
+    tw[k] = make_double2(c, s);
+  }
+  for (int i = tid; i < g.dx; i += kF64Threads) x[i] = make_double2(0.0, 0.0);
+  if (tid == 0) red[0] = 0;
+  __syncthreads();
+
+  // SoS table exactly as the fused kernel builds it (fading.py:326-342, :492-499)
+  {
+    const int ns = a.sinusoids;
+    const int o_los_ang = 100, o_nlos_ang = 100 + L, o_los_ph = 100 + L + L * ns, o_nlos_ph = 100 + 2 * L + L * ns;
+    const double inv_fs = 1.0 / a.bandwidth;
+    for (int e = tid; e < L * a.terms; e += kF64Threads) {
+      const int p = e / a.terms, k = e - p * a.terms;
+      double* row = sos + 3 * e;
+      if (k < ns) {
+        const double ang = -M_PI + 2.0 * M_PI * uni[o_nlos_ang + p * ns + k];
+        row[0] = a.doppler * cos((2.0 * M_PI * (k + 1) + ang) / ns) * inv_fs;
+        row[1] = -M_PI + 2.0 * M_PI * uni[o_nlos_ph + p * ns + k];
+        row[2] = a.path_gains[L + p];
+      } else {
+        const double ang = 2.0 * M_PI * uni[o_los_ang + p];
+        row[0] = a.los_doppler * cos(ang) * inv_fs;
+        row[1] = -M_PI + 2.0 * M_PI * uni[o_los_ph + p];
+        row[2] = a.path_gains[p];
+      }
+    }
+  }
+  // spatial response of the single antenna pair: exp(2 pi j u_00) (fading.py:476-489)
+  double2 S;
+  sincospi(2.0 * uni[0], &S.y, &S.x);
+  __syncthreads();
+  for (int e = tid; e < L * a.words; e += kF64Threads) {
+    const int p = e / a.words, w = e - p * a.words;
+    abar[e] = path_in_range(a, p)
+                  ? tap_mean(sos + 3 * p * a.terms, a.terms, static_cast<double>(w) * period + a.cp, n)
+                  : make_double2(0.0, 0.0);
+  }
+
+  // transmit: map, inverse DFT (unnormalised, the constellation carries sqrt(P / N)), prefix
+  for (int w = 0; w < a.words; ++w) {
+    for (int q = tid; q < n; q += kF64Threads) {
+      const int bin = subcarrier_bin(q, n, a.dc);
+      const uint32_t v = symbol_bits(bits, (w * n + q) * a.bps, a.bps);
+      const double2 p = constellation_point(a.order, 1, bits_to_label(v, a.bps));
+      sym[bin] = make_double2(p.x * a.tx_scale, p.y * a.tx_scale);
+    }
+    __syncthreads();
+    double2* xw = x + g.dx + w * period;
+    for (int k = tid; k < n; k += kF64Threads) {
+      double re = 0.0, im = 0.0;
+      for (int b = 0; b < n; ++b) {
+        const double2 t = tw[(b * k) & (n - 1)];  // exp(+2 pi j b k / N) = conj(t)
+        re += sym[b].x * t.x + sym[b].y * t.y;
+        im += sym[b].y * t.x - sym[b].x * t.y;
+      }
+      xw[a.cp + k] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int i = tid; i < a.cp; i += kF64Threads) xw[i] = xw[n + i];
+    __syncthreads();
+  }
+
+  // channel: y_n = S sum_l h_l(n - d_l) x_(n - d_l) + sqrt(P / 2) (N + jN) over the frame
+  const double np = a.noise_power[local_drop];
+  const double sigma = sqrt(0.5 * np);
+  for (int i = tid; i < g.frame; i += kF64Threads) {
+    double zr = 0.0, zi = 0.0;
+    for (int p = 0; p < L; ++p) {
+      if (!path_in_range(a, p)) continue;
+      const int d = a.delays[p];
+      const int m = i - d;
+      if (m < 0) continue;
+      double hr = 0.0, hi = 0.0;
+      const double* row = sos + 3 * p * a.terms;
+      for (int k = 0; k < a.terms; ++k) {
+        if (row[3 * k + 2] == 0.0) continue;
+        double s, c;
+        sincos(reduce_phase(fma(row[3 * k], static_cast<double>(m), row[3 * k + 1])), &s, &c);
+        hr += row[3 * k + 2] * c;
+        hi += row[3 * k + 2] * s;
+      }
+      const double2 xv = x[g.dx + m];
+      zr += hr * xv.x - hi * xv.y;
+      zi += hr * xv.y + hi * xv.x;
+    }
+    // noise sample i: words (0, 1) of call i / 2 for even i, (2, 3) for odd i
+    const uint4 wn = stream_call(a.seed, drop, HPX_STREAM_NOISE, static_cast<uint32_t>(i >> 1));
+    const uint32_t w0 = (i & 1) ? wn.z : wn.x, w1 = (i & 1) ? wn.w : wn.y;
+    const double u1 = (static_cast<double>(w0 >> 8) + 0.5) * 5.9604644775390625e-08;
+    const double u2 = static_cast<double>(w1 >> 8) * 5.9604644775390625e-08;
+    const double rho = sqrt(-2.0 * log(u1));
+    double ns, nc;
+    sincospi(2.0 * u2, &ns, &nc);
+    y[i] = make_double2(S.x * zr - S.y * zi + sigma * rho * nc, S.x * zi + S.y * zr + sigma * rho * ns);
+  }
+  __syncthreads();
+
+  // receive: DFT of each word's data window, ideal CSI, ZF / MMSE, demap, bit errors
+  const double rx_scale = 1.0 / sqrt(static_cast<double>(n));
+  const double s2 = a.detector == HPX_DETECT_MMSE ? np : 0.0;
+  int errors = 0;
+  for (int w = 0; w < a.words; ++w) {
+    const double2* yw = y + w * period + a.cp;
+    for (int q = tid; q < n; q += kF64Threads) {
+      const int bin = subcarrier_bin(q, n, a.dc);
+      double yr = 0.0, yi = 0.0;
+      for (int k = 0; k < n; ++k) {
+        const double2 t = tw[(bin * k) & (n - 1)];
+        yr += yw[k].x * t.x - yw[k].y * t.y;
+        yi += yw[k].x * t.y + yw[k].y * t.x;
+      }
+      yr *= rx_scale;
+      yi *= rx_scale;
+      double cr = 0.0, ci = 0.0;
+      for (int p = 0; p < L; ++p) {
+        const int d = a.delays[p];
+        if (!path_in_range(a, p) || d >= a.keep_below) continue;
+        const double2 m = abar[p * a.words + w], t = tw[(bin * d) & (n - 1)];
+        cr += m.x * t.x - m.y * t.y;
+        ci += m.x * t.y + m.y * t.x;
+      }
+      const double hr = S.x * cr - S.y * ci, hi = S.x * ci + S.y * cr;
+      const double den = hr * hr + hi * hi + s2;
+      const double gain = den > 0.0 ? 1.0 / den : 0.0;  // pinv of a zero channel is zero
+      const double xr = (hr * yr + hi * yi) * gain, xi = (hr * yi - hi * yr) * gain;
+      const uint32_t rx = demap_symbol(a.mod, xr, xi);
+      const uint32_t tx = symbol_bits(bits, (w * n + q) * a.bps, a.bps);
+      errors += __popc(tx ^ rx);
+      if (equalized) equalized[local_drop * static_cast<int64_t>(a.words) * n + w * n + q] = make_double2(xr, xi);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) errors += __shfl_down(errors, off, 64);
+  if ((tid & 63) == 0) atomicAdd(red, errors);
+  __syncthreads();
+  if (tid == 0) a.errors[local_drop] = red[0];
+}
+
+}  // namespace
+
+extern "C" int hpx_run_batch_f64(const hpx_link* link_host, const int32_t* delays, const double* path_gains,
+                                 uint64_t seed, int64_t first_drop, int64_t num_drops, const double* noise_power,
+                                 int32_t* errors, hpx_c128* equalized, void* stream) {
+  HPX_REQUIRE(link_host && delays && path_gains && noise_power && errors, "hpx_run_batch_f64: null pointer");
+  HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff && first_drop >= 0 &&
+                  first_drop + num_drops <= (int64_t(1) << 32),
+              "hpx_run_batch_f64: invalid drop range");
+  if (link_host->num_tx != 1 || link_host->num_rx != 1) {
+    set_error("hpx_run_batch_f64: single-antenna links only (got %dx%d)", link_host->num_rx, link_host->num_tx);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  if (link_host->num_subcarriers > kF64MaxSubcarriers) {
+    set_error("hpx_run_batch_f64: at most %d subcarriers (got %d)", kF64MaxSubcarriers, link_host->num_subcarriers);
+    return HPX_ERR_UNSUPPORTED;
+  }
+  Plan plan;
+  const int st = make_plan(link_host, nullptr, &plan);
+  if (st != HPX_OK) return st;
+  if (num_drops == 0) return HPX_OK;
+  BatchArgs& a = plan.args;
+  a.delays = delays;
+  a.path_gains = path_gains;
+  a.noise_power = noise_power;
+  a.errors = errors;
+  a.seed = seed;
+  a.first_drop = first_drop;
+  F64Layout g{};
+  g.dx = a.maxd;
+  g.frame = a.words * (a.n + a.cp);
+  g.bits_words = static_cast<int>((static_cast<int64_t>(a.words) * a.n * a.bps + 31) / 32) + 1;
+  int64_t o = 0;
+  auto take = [&](int64_t bytes) {
+    const int at = static_cast<int>(o);
+    o += align16(bytes);
+    return at;
+  };
+  g.off_uni = take(static_cast<int64_t>(a.nnorm) * sizeof(double));
+  g.off_sos = take(static_cast<int64_t>(a.paths) * a.terms * 3 * sizeof(double));
+  g.off_tw = take(static_cast<int64_t>(a.n) * sizeof(double2));
+  g.off_sym = take(static_cast<int64_t>(a.n) * sizeof(double2));
+  g.off_x = take(static_cast<int64_t>(g.dx + g.frame) * sizeof(double2));
+  g.off_y = take(static_cast<int64_t>(g.frame) * sizeof(double2));
+  g.off_abar = take(static_cast<int64_t>(a.paths) * a.words * sizeof(double2));
+  g.off_bits = take(static_cast<int64_t>(g.bits_words) * sizeof(uint32_t));
+  g.off_red = take(16);
+  g.bytes = o;
+  if (g.bytes > kMaxLds) {
+    set_error("hpx_run_batch_f64: %lld bytes of LDS per drop exceed a CU's %lld", static_cast<long long>(g.bytes),
+              static_cast<long long>(kMaxLds));
+    return HPX_ERR_UNSUPPORTED;
+  }
+  if (!ensure_lds(reinterpret_cast<const void*>(f64_drop_kernel), g.bytes)) {
+    set_error("hpx_run_batch_f64: %lld bytes of LDS per workgroup not available", static_cast<long long>(g.bytes));
+    return HPX_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL(f64_drop_kernel, dim3(static_cast<unsigned>(num_drops)), dim3(kF64Threads),
+                     static_cast<size_t>(g.bytes), static_cast<hipStream_t>(stream), a,
+                     g, reinterpret_cast<double2*>(equalized));
+  return check_launch("hpx_run_batch_f64");
+}

@@ -134,6 +134,31 @@ class BatchedLink:
             ctypes.byref(out), _lib.stream_handle() if stream is None else stream)
         _lib.check(st)
 
+    def run_f64(self, seed: int, first_drop: int, noise_power: torch.Tensor, errors: torch.Tensor,
+                equalized: torch.Tensor | None = None) -> None:
+        """The float64 counterpart of ``run`` for single-antenna links of up to 256 subcarriers
+        without a frame or antenna correlation (``hpx_run_batch_f64``): the same drops with every
+        value a double and the taps summed exactly per sample.  equalized: optional complex128
+        (drops, words * N) in grid order."""
+
+        lk = self.link
+        n = int(noise_power.numel())
+        if errors.numel() < n or errors.dtype != torch.int32:
+            raise ValueError("errors must be an int32 tensor with one entry per drop")
+        if noise_power.dtype != torch.float64:
+            raise ValueError("noise_power must be float64")
+        if self.c_frame is not None or self.corr_rx is not None:
+            raise ValueError("the float64 path takes links without a frame or antenna correlation")
+        per_drop = lk.words * lk.num_subcarriers
+        if equalized is not None and (equalized.dtype != torch.complex128 or not equalized.is_contiguous()
+                                      or equalized.numel() < n * per_drop):
+            raise ValueError(f"equalized must be a contiguous complex128 tensor with {per_drop} entries per drop")
+        lib = _lib.load()
+        _lib.check(lib.hpx_run_batch_f64(ctypes.byref(self.c_link), _lib.ptr(self.delays), _lib.ptr(self.path_gains),
+                                         ctypes.c_uint64(seed & (2**64 - 1)), int(first_drop), n,
+                                         _lib.ptr(noise_power), _lib.ptr(errors), _lib.ptr(equalized),
+                                         _lib.stream_handle()))
+
 
 class BatchedSimulation:
     """Monte Carlo BER campaign over an SNR grid, ``drops_per_launch`` drops per fused launch.

@@ -500,6 +500,17 @@ int hpx_massive_run(const hpx_massive_link* link_host, const int32_t* delays, co
 int hpx_spatial_detector_host(int num_rx, int num_tx, const hpx_c128* spatial_host, double* lambda_host,
                               hpx_c128* v_host, hpx_c128* a_host);
 
+/* Float64 drops of a single-antenna link of at most 256 subcarriers without a frame or antenna
+ * correlation: the drops of hpx_run_batch (same seed, drop indices and streams; the noise from the
+ * same 24-bit uniforms) with every value a double, the taps summed exactly per sample
+ * (hermespy/channel/fading/fading.py:293-343 in float64, as the reference computes them) and the
+ * OFDM transforms as direct DFTs.  Replaces the same call chain as hpx_run_batch for the float64
+ * measurement of the small link.  equalized: complex128 [num_drops][words * num_subcarriers]
+ * (element q of word w at w * N + q) or NULL. */
+int hpx_run_batch_f64(const hpx_link* link_host, const int32_t* delays, const double* path_gains, uint64_t seed,
+                      int64_t first_drop, int64_t num_drops, const double* noise_power, int32_t* errors,
+                      hpx_c128* equalized, void* stream);
+
 /* Dynamic LDS bytes one hpx_run_batch workgroup needs for `link` (0 if unsupported). */
 int64_t hpx_run_batch_lds_bytes(const hpx_link* link_host);
 

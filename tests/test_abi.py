@@ -135,3 +135,28 @@ def test_new_entry_points_validate_without_a_device(native_lib):
     with pytest.raises(ValueError, match="null input or output"):
         _lib.check(native_lib.hpx_run_batch_drops(ctypes.byref(link), None, None, None, None, None, None, 1, 0, 1,
                                                   ctypes.byref(out), None))
+
+
+def test_float64_path_validates_without_a_device(native_lib):
+    """hpx_run_batch_f64 refuses links it does not cover before any device call."""
+    import ctypes
+
+    from hermespy_amd import _lib
+    from hermespy_amd.configs import mimo4x4_1024, siso_64
+    from hermespy_amd.uneven import _c_link
+
+    big = _c_link(mimo4x4_1024())
+    with pytest.raises(ValueError, match="single-antenna"):
+        _lib.check(native_lib.hpx_run_batch_f64(ctypes.byref(big), 0x10, 0x20, 1, 0, 4, 0x30, 0x40, None, None))
+    cfg = siso_64()
+    wide = _c_link(cfg)
+    wide.num_subcarriers = 512
+    with pytest.raises(ValueError, match="at most 256"):
+        _lib.check(native_lib.hpx_run_batch_f64(ctypes.byref(wide), 0x10, 0x20, 1, 0, 4, 0x30, 0x40, None, None))
+    small = _c_link(cfg)
+    with pytest.raises(ValueError, match="null"):
+        _lib.check(native_lib.hpx_run_batch_f64(ctypes.byref(small), None, 0x20, 1, 0, 4, 0x30, 0x40, None, None))
+    with pytest.raises(ValueError, match="drop range"):
+        _lib.check(native_lib.hpx_run_batch_f64(ctypes.byref(small), 0x10, 0x20, 1, 2**32 - 1, 4, 0x30, 0x40, None,
+                                                None))
+    assert native_lib.hpx_run_batch_f64(ctypes.byref(small), 0x10, 0x20, 1, 0, 0, 0x30, 0x40, None, None) == 0
