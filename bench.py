@@ -290,6 +290,49 @@ def massive_leg(drops: int = 256, chunk: int = 128) -> dict:
             "bits_per_s": drops * cfg.bits_per_drop / (ms / 1e3), "snr_db": snr.tolist(), "ber": ber}
 
 
+def small_link_leg(drops_per_point: int = 1000, reps: int = 3) -> dict:
+    """The small single-antenna link (64 subcarriers, QPSK, ZF, SNR 0..20 dB in 5 dB steps, 1000
+    drops each, one launch): the float64 path and the float32 fused path, and the float32
+    path's error against the float64 one on the same drops."""
+
+    import torch
+
+    from hermespy_amd.configs import siso_64
+    from hermespy_amd.simulation import BatchedLink
+
+    cfg = siso_64()
+    snr = np.array([0.0, 5.0, 10.0, 15.0, 20.0])
+    n = drops_per_point * snr.size
+    idx = np.arange(n) // drops_per_point
+    bl = BatchedLink(cfg)
+    npw = torch.as_tensor(cfg.noise_power(snr[idx]), dtype=torch.float64, device="cuda")
+    per_drop = cfg.words * cfg.num_subcarriers
+    e64 = torch.zeros(n, dtype=torch.int32, device="cuda")
+    e32 = torch.zeros(n, dtype=torch.int32, device="cuda")
+    q64 = torch.empty((n, per_drop), dtype=torch.complex128, device="cuda")
+    q32 = torch.empty((n, 1, per_drop), dtype=torch.complex64, device="cuda")
+    bl.run_f64(SEED, 0, npw, e64, q64)
+    bl.run(SEED, 0, npw, e32, q32)
+    torch.cuda.synchronize()
+    err = np.linalg.norm((q32[:, 0].to(torch.complex128) - q64).cpu().numpy(), axis=1)
+    rel = err / np.linalg.norm(q64.cpu().numpy(), axis=1)
+    res = {"workload": f"{cfg.name}, 5 SNR points x {drops_per_point} drops", "drops": n,
+           "f32_rel_error_vs_f64": {"max": float(rel.max()), "median": float(np.median(rel))},
+           "error_counts_equal": int((e64 == e32).sum().item())}
+    for name, launch in (("f64", lambda: bl.run_f64(SEED, 0, npw, e64)), ("f32", lambda: bl.run(SEED, 0, npw, e32))):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            launch()
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        e = (e64 if name == "f64" else e32).cpu().numpy()
+        res[name] = {"ms": ms, "drops_per_s": n / (ms / 1e3),
+                     "ber": [float(e[idx == k].sum() / (drops_per_point * cfg.bits_per_drop)) for k in range(snr.size)]}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +346,7 @@ def main() -> int:
     ap.add_argument("--no-estimators", action="store_true", help="skip the pilot-estimation leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the heterogeneous-batch leg")
     ap.add_argument("--no-massive", action="store_true", help="skip the massive-MIMO leg")
+    ap.add_argument("--no-small", action="store_true", help="skip the float64 / float32 small-link leg")
     ap.add_argument("--no-campaign", action="store_true", help="skip the early-stopping campaign leg")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (counter calibration for profiles)")
@@ -393,6 +437,8 @@ def main() -> int:
         out["uneven_batch"] = uneven_leg()
     if not args.no_massive:
         out["massive_mimo"] = massive_leg()
+    if not args.no_small:
+        out["small_link"] = small_link_leg()
     if not args.no_campaign:
         out["campaign"] = campaign_leg()
     if not args.no_cpu_baseline:
