@@ -55,7 +55,7 @@ def _oracle_drops(args) -> int:
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from hermespy_amd.configs import CONFIGS
-    from oracle import drop as odrop  # the checker, timed as the CPU baseline
+    from oracle import drop as odrop  # the oracle, timed as the CPU baseline
 
     cfg = CONFIGS[cfg_name]()
     link = odrop.Link(cfg.num_rx, cfg.num_tx, cfg.num_subcarriers, cfg.words, cfg.prefix_ratio, cfg.modulation_order,
@@ -353,7 +353,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (drops shard across GPUs by "
-              f"Philox counter range; multi-GPU runs are not part of this build)", file=sys.stderr)
+              f"Philox counter range; this benchmark runs on one GPU)", file=sys.stderr)
         return 2
 
     import torch

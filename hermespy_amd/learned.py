@@ -18,7 +18,7 @@ gradients, batch-NMSE loss and Adam, on drops the fused simulator generates
 
 It replaces the interpolation step of ``OrthogonalLeastSquaresChannelEstimation``
 (``hermespy/modem/waveforms/orthogonal/channel.py:29-66``); the LMMSE estimator
-of :mod:`hermespy_amd.lmmse` is its fallback and the judge of its NMSE.
+of :mod:`hermespy_amd.lmmse` is its fallback and the baseline its NMSE is measured against.
 """
 
 from __future__ import annotations

@@ -9,7 +9,7 @@ mapping/demapping (``hermespy/modem/tools/psk_qam_mapping.py``), ideal CSI
 bit error evaluator (``hermespy/modem/evaluators.py``).  Each function cites the
 reference file:line it follows.
 
-It is the checker of the HIP path, never the product: only ``tests/``,
+It checks the HIP path and is never the product: only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
 import it.  The product path (``hermespy_amd``) never calls into it.
 

@@ -1,7 +1,7 @@
 """Massive-MIMO uplink drop (many receive antennas, single-antenna users), float64.
 
-Test infrastructure: only ``tests/`` use this module, as the checker of the
-massive-MIMO device path.
+Test infrastructure: only ``tests/`` use this module, to check the
+massive-MIMO device path against it.
 
 The reference caps antenna arrays at 10 elements: ``MultipathFadingRealization``
 draws one 10 x 10 block of uniforms for the spatial response and slices it

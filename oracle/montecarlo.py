@@ -1,8 +1,8 @@
 """Monte Carlo grid-section control, sequential and in float64: the confidence-based
 early stopping of ``ScalarEvaluationResult`` and the per-section sample loop.
 
-Test infrastructure: only ``tests/`` use this module, as the checker of
-``hermespy_amd.campaign``.
+Test infrastructure: only ``tests/`` use this module, to check
+``hermespy_amd.campaign`` against it.
 
 Restated from:
 
