@@ -46,6 +46,32 @@ def test_f64_matches_oracle(cuda, detector):
     assert err.sum() > 0
 
 
+def test_f64_rician_wide_link_matches_oracle(cuda):
+    """256 subcarriers without DC suppression, 9 paths with a line-of-sight first tap (K = 0 dB),
+    16-QAM, MMSE: the float64 path against the float64 oracle."""
+
+    from hermespy_amd.simulation import BatchedLink
+    from hermespy_amd.uneven import DropShape
+
+    base = DropShape(1, 256, 9, 2.0, 300.0, 16, 15.0).link()
+    cfg = dataclasses.replace(base, rice_factors=np.array([1.0] + [0.0] * 8), dc_suppression=False)
+    print("float64 1x1-256 Rician vs oracle", flush=True)
+    snrs = np.array([6.0, 12.0, 18.0, 24.0])
+    bl = BatchedLink(cfg)
+    npw = torch.as_tensor(cfg.noise_power(snrs), dtype=torch.float64, device=cuda)
+    err = torch.empty(snrs.size, dtype=torch.int32, device=cuda)
+    eq = torch.empty((snrs.size, cfg.words * cfg.num_subcarriers), dtype=torch.complex128, device=cuda)
+    bl.run_f64(SEED, 7, npw, err, eq)
+    torch.cuda.synchronize()
+    err, eq = err.cpu().numpy(), eq.cpu().numpy()
+    link = oracle_link(cfg)
+    for i, snr in enumerate(snrs):
+        ref = odrop.run_drop(link, SEED, 7 + i, float(snr), faithful=True)
+        e = rel_err(eq[i], ref.equalized[0])
+        assert e < 1e-11, (i, snr, e)
+        assert int(err[i]) == ref.errors, (i, int(err[i]), ref.errors)
+
+
 def test_f64_rejects_other_links(cuda):
     from hermespy_amd.configs import mimo2x2_512_pilots
     from hermespy_amd.simulation import BatchedLink
