@@ -45,12 +45,14 @@ def algorithmic_flops_per_drop(cfg) -> float:
     return 8.0 * (tdl + smix + csi + det) + fft
 
 
-def _oracle_drops(args) -> int:
+def _oracle_drops(args):
     """Worker: oracle drops for budget_s seconds on one core (reference-faithful unless faithful is
-    False: closed-form CSI and vectorised taps); returns the count."""
+    False: closed-form CSI and vectorised taps); returns the count, or with keep > 0 the count and
+    (errors, bits, equalized) of the first `keep` drops, which it runs in any case."""
 
     cfg_name, first, stride, budget_s = args[:4]
     faithful = args[4] if len(args) > 4 else True
+    keep = args[5] if len(args) > 5 else 0
     from threadpoolctl import threadpool_limits
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -63,30 +65,50 @@ def _oracle_drops(args) -> int:
                       cfg.corr_rx, cfg.corr_tx, cfg.detector, cfg.dc_suppression, cfg.gain, cfg.device_power)
     snr = np.asarray(cfg.snr_db)
     done = 0
+    kept = []
     with threadpool_limits(1):
         t0 = time.perf_counter()
-        while time.perf_counter() - t0 < budget_s or done == 0:
+        while time.perf_counter() - t0 < budget_s or done == 0 or done < keep:
             d = first + done * stride
-            odrop.run_drop(link, SEED, d, float(snr[d % snr.size]), faithful=faithful)
+            r = odrop.run_drop(link, SEED, d, float(snr[d % snr.size]), faithful=faithful)
+            if done < keep:
+                kept.append((r.errors, r.bits, r.equalized))
             done += 1
-    return done
+    return (done, kept) if keep else done
 
 
 def cpu_baseline(cfg, budget_s: float = 15.0, all_core_budget_s: float = 10.0, workers: int = 16,
-                 vectorised_budget_s: float = 5.0) -> dict:
+                 vectorised_budget_s: float = 5.0, gpu_drops=None, check_drops: int = 11) -> dict:
     """Reference-faithful float64 oracle (dense impulse response, per-path loop, pinv/solve): one host core,
     then one process per core for the all-core aggregate (as the reference runs one actor per CPU), and
-    the oracle's vectorised mode (closed-form CSI, no dense impulse response) on one core for fairness."""
+    the oracle's vectorised mode (closed-form CSI, no dense impulse response) on one core for fairness.
+
+    gpu_drops(k) -> (errors, equalized) of the fused path's drops 0 .. k-1 (the benchmark's drops):
+    the first `check_drops` oracle drops of the single-core leg (one per SNR point) check them, giving
+    the run's error against the float64 oracle (``fp64_check``)."""
 
     from concurrent.futures import ProcessPoolExecutor
 
     name = "mimo4x4-1024"
+    keep = check_drops if gpu_drops is not None else 0
     t0 = time.perf_counter()
-    done = _oracle_drops((name, 0, 1, budget_s))
+    res = _oracle_drops((name, 0, 1, budget_s, True, keep))
     dt = time.perf_counter() - t0
+    done, kept = res if keep else (res, [])
     out = {"value": done / dt, "unit": "drops/s", "cores": 1, "kind": "port",
            "sample": f"{done} drops of {cfg.name}, reference-faithful float64 oracle (dense CIR, per-path loop), "
                      f"{dt:.1f} s on 1 host core of {os.cpu_count()}"}
+    if keep:
+        g_err, g_eq = gpu_drops(keep)
+        rel = [float(np.linalg.norm(g_eq[i] - eq) / np.linalg.norm(eq)) for i, (_, _, eq) in enumerate(kept)]
+        d_err = [int(g_err[i]) - int(e) for i, (e, _, _) in enumerate(kept)]
+        bits = sum(b for _, b, _ in kept)
+        out["fp64_check"] = {"drops": keep, "snr_db": [float(x) for x in np.asarray(cfg.snr_db)[np.arange(keep) % len(cfg.snr_db)]],
+                             "max_rel_err_equalized": max(rel), "median_rel_err_equalized": float(np.median(rel)),
+                             "delta_errors": d_err, "bits": int(bits),
+                             "ber_gpu": float(sum(int(x) for x in g_err[:keep])) / bits,
+                             "ber_fp64_oracle": float(sum(e for e, _, _ in kept)) / bits,
+                             "oracle": "reference-faithful float64 (dense impulse response, pinv / stacked solve)"}
     workers = max(1, min(workers, os.cpu_count() or 1))
     if all_core_budget_s > 0 and workers > 1:
         t0 = time.perf_counter()
@@ -442,7 +464,21 @@ def main() -> int:
     if not args.no_campaign:
         out["campaign"] = campaign_leg()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
+        def gpu_drops(k: int):
+            """The benchmark's drops 0 .. k-1 again, with their equalised symbols exported."""
+
+            e = torch.empty(k, dtype=torch.int32, device=dev)
+            q = torch.empty((k, cfg.num_tx, cfg.num_data), dtype=torch.complex64, device=dev)
+            link.run(SEED, 0, npw[0][:k].contiguous(), e, q)
+            torch.cuda.synchronize()
+            assert np.array_equal(e.cpu().numpy(), errors[0, :k].cpu().numpy())  # the timed launches' drops
+            return e.cpu().numpy(), q.cpu().numpy().astype(np.complex128)
+
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget, gpu_drops=gpu_drops)
+        if "fp64_check" in out["cpu_baseline"]:
+            chk = out["cpu_baseline"]["fp64_check"]
+            out["error_vs_fp64_oracle"] = {k: chk[k] for k in ("drops", "max_rel_err_equalized", "delta_errors",
+                                                              "ber_gpu", "ber_fp64_oracle")}
     print(json.dumps(out), flush=True)
     return 0
 

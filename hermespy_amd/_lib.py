@@ -118,6 +118,8 @@ SIGNATURES = {
     "hpx_fading_taps": (c_int, [c_vp, c_int, c_int, c_i64, c_vp, c_vp]),
     "hpx_tdl_propagate": (c_int, [c_vp, c_int, c_i64, c_i64, c_vp, c_int, c_int, c_dbl, c_vp, c_i32, c_vp, c_int,
                                   c_vp, c_i64, c_vp]),
+    "hpx_fading_state": (c_int, [c_vp, c_int, c_int, c_vp, c_i32, c_vp, c_int, c_int, c_i64, c_vp, c_vp]),
+    "hpx_csi_propagate": (c_int, [c_vp, c_int, c_int, c_i64, c_i32, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
     "hpx_awgn": (c_int, [c_vp, c_int, c_i64, c_i64, c_u64, c_u32, c_dbl, c_vp]),
     "hpx_map_symbols": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp]),
     "hpx_demap_hard": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp]),
@@ -223,11 +225,15 @@ def check_count(value: int) -> int:
     return int(value)
 
 
-def ptr(t) -> int | None:
-    """Device (or host) address of a tensor, None for None."""
+def ptr(t, name: str = "tensor") -> int | None:
+    """Device address of a tensor for an array argument of the library, None for None.
+
+    Every such argument is a device pointer: a host tensor would hand the kernels a host
+    address, so it is rejected (ValueError) before any launch, as is a non-contiguous view."""
 
     if t is None:
         return None
+    require_device(t, name)
     return t.data_ptr()
 
 

@@ -116,8 +116,6 @@ class ScalarEvaluationResult:
         self.count = torch.zeros(self.num_sections, dtype=torch.int64, device=dev)
         self.errors = torch.zeros_like(self.count)
         self.bits = torch.zeros_like(self.count)
-        self._c_stats = _lib.HpxSectionStats(*[_lib.ptr(t) for t in (self.sum, self.sum_sq, self.count, self.errors,
-                                                                      self.bits)])
 
     @property
     def title(self) -> str:
@@ -140,9 +138,11 @@ class ScalarEvaluationResult:
                 raise ValueError(f"{name} must be a contiguous int32 tensor with one entry per drop")
         if bits_per_drop is None and (bits is None or bits <= 0):
             raise ValueError("give the bits per drop")
+        stats = _lib.HpxSectionStats(*[_lib.ptr(t, "section counter") for t in (self.sum, self.sum_sq, self.count,
+                                                                                self.errors, self.bits)])
         _lib.check(_lib.load().hpx_section_accumulate(
-            _lib.ptr(errors), _lib.ptr(section), int(first_drop), _lib.ptr(bits_per_drop),
-            int(bits or 0), n, self.num_sections, ctypes.byref(self._c_stats),
+            _lib.ptr(errors, "errors"), _lib.ptr(section, "section"), int(first_drop),
+            _lib.ptr(bits_per_drop, "bits_per_drop"), int(bits or 0), n, self.num_sections, ctypes.byref(stats),
             _lib.stream_handle() if stream is None else stream))
 
     def add_artifact(self, coordinates, artifact, compute_confidence: bool = True) -> bool:
@@ -303,7 +303,8 @@ class SectionQueue:
 
     Per launch, every active section (in grid order) receives
     ``min(num_samples - issued, min_num_samples - issued % min_num_samples, cap)`` drops, with
-    ``cap = max(1, max_drops // active)``; a section leaves the queue when ``num_samples``
+    ``cap = max(1, max_drops // active)``; with more active sections than ``max_drops`` a launch
+    serves ``max_drops`` of them, one drop each, round robin; a section leaves the queue when ``num_samples``
     drops were issued (``actors.py:102-106``) or when :meth:`disable_section` is called."""
 
     def __init__(self, num_sections: int, num_samples: int, min_num_samples: int = 1024,
@@ -316,6 +317,7 @@ class SectionQueue:
         self.max_drops = int(max_drops)
         self.issued = np.zeros(self.num_sections, dtype=np.int64)
         self.active = np.ones(self.num_sections, dtype=bool)
+        self._cursor = 0  # round-robin start when more sections are active than drops fit a launch
 
     @property
     def num_active(self) -> int:
@@ -331,6 +333,13 @@ class SectionQueue:
         act = np.flatnonzero(self.active)
         if act.size == 0:
             return np.zeros(0, dtype=np.int32), np.zeros(0, dtype=np.int64)
+        if act.size > self.max_drops:
+            # more active sections than drops per launch: one drop each for the next max_drops
+            # sections in round-robin order, starting after the last section served
+            start = int(np.searchsorted(act, self._cursor)) % act.size
+            chosen = np.roll(act, -start)[: self.max_drops]
+            self._cursor = int(chosen[-1]) + 1
+            act = np.sort(chosen)
         cap = max(1, self.max_drops // act.size)
         issued = self.issued[act]
         block = np.minimum(np.minimum(self.num_samples - issued, self.min_num_samples - issued % self.min_num_samples),

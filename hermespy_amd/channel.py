@@ -20,6 +20,8 @@ import torch
 from scipy.special import ndtr
 
 from . import _lib
+from .core import (ChannelStateFormat, ChannelStateInformation, InterpolationMode, LinkState, Signal,
+                   SimulatedDevice, device_state)
 
 
 class AntennaCorrelation:
@@ -68,7 +70,8 @@ class MultipathFadingSample:
     """One sampled link of a fading realisation (``fading.py:147-406``)."""
 
     def __init__(self, power_profile, delay_profile, los_angles, nlos_angles, los_phases, nlos_phases, los_gains,
-                 nlos_gains, los_doppler, nlos_doppler, spatial_response, gain, bandwidth, device="cuda") -> None:
+                 nlos_gains, los_doppler, nlos_doppler, spatial_response, gain, bandwidth, device="cuda",
+                 carrier_frequency: float = 0.0) -> None:
         self.power_profile = np.asarray(power_profile, dtype=np.float64)
         self.delay_profile = np.asarray(delay_profile, dtype=np.float64)
         self.los_angles = np.asarray(los_angles, dtype=np.float64)
@@ -82,6 +85,7 @@ class MultipathFadingSample:
         self.spatial_response = np.asarray(spatial_response, dtype=np.complex128)
         self.gain = float(gain)
         self.bandwidth = float(bandwidth)
+        self.carrier_frequency = float(carrier_frequency)
         self.device = torch.device(device)
         self._dev_cache: dict | None = None
 
@@ -153,13 +157,9 @@ class MultipathFadingSample:
                                                int(num_samples), _lib.ptr(out), _lib.stream_handle()))
         return out
 
-    def propagate(self, signal) -> torch.Tensor:
+    def _propagate(self, x: torch.Tensor) -> torch.Tensor:
         """``_propagate`` (``fading.py:371-406``): (Ntx, M) -> (Nrx, M + max delay), complex64 on the device."""
 
-        x = _as_device_signal(signal, self.device)
-        if x.shape[0] != self.num_transmit_antennas:
-            raise ValueError(f"Number of signal streams ({x.shape[0]}) does not match the number of transmit "
-                             f"antennas ({self.num_transmit_antennas})")
         d = self._device_tables()
         m = int(x.shape[1])
         dmax = self.max_delay_samples
@@ -172,18 +172,42 @@ class MultipathFadingSample:
             _lib.ptr(y), m + dmax, _lib.stream_handle()))
         return y
 
-    def state(self, num_samples: int, max_num_taps: int) -> torch.Tensor:
-        """Dense impulse response (Nrx, Ntx, num_samples, taps) (``fading.py:345-369``)."""
+    def propagate(self, signal, interpolation_mode: InterpolationMode = InterpolationMode.NEAREST):
+        """``ChannelSample.propagate`` (``channel.py:306-379``) over ``_propagate``: a ``Signal``
+        gives a ``Signal`` at the sample's bandwidth (empty when the expected energy scale is
+        zero); a bare (Ntx, M) array or tensor gives the (Nrx, M + max delay) tensor."""
+
+        if isinstance(signal, Signal):
+            if self.expected_energy_scale <= 0.0:
+                return Signal.Empty(signal.sampling_rate, self.num_receive_antennas, 0, signal.carrier_frequency,
+                                    signal.noise_power, signal.delay, device=self.device)
+            if signal.num_streams != self.num_transmit_antennas:
+                raise ValueError("Number of signal streams to be propagated does not match the number of transmitter "
+                                 f"antennas ({signal.num_streams} != {self.num_transmit_antennas}))")
+            x = _as_device_signal(signal.samples, self.device)
+            return Signal(self._propagate(x), self.bandwidth, self.carrier_frequency, signal.noise_power, signal.delay)
+        x = _as_device_signal(signal, self.device)
+        if x.shape[0] != self.num_transmit_antennas:
+            raise ValueError(f"Number of signal streams ({x.shape[0]}) does not match the number of transmit "
+                             f"antennas ({self.num_transmit_antennas})")
+        return self._propagate(x)
+
+    def state(self, num_samples: int, max_num_taps: int,
+              interpolation_mode: InterpolationMode = InterpolationMode.NEAREST) -> ChannelStateInformation:
+        """Dense impulse response (Nrx, Ntx, num_samples, taps) (``fading.py:345-369``): taps =
+        min(1 + max delay, max_num_taps); a path whose delay exceeds the last tap is left out
+        (``hpx_fading_state``)."""
 
         num_taps = min(1 + round(float(self.delay_profile.max()) * self.bandwidth), int(max_num_taps))
-        h = self.taps(num_samples)
-        siso = torch.zeros((int(num_samples), num_taps), dtype=torch.complex64, device=self.device)
-        for p, dl in enumerate(self.delay_samples):
-            if dl > num_taps:
-                continue
-            siso[:, dl] += h[p]
-        s = torch.as_tensor(self.spatial_response.astype(np.complex64), device=self.device)
-        return s[:, :, None, None] * siso[None, None, :, :]
+        if num_taps < 1:
+            raise ValueError("the channel state needs at least one tap")
+        d = self._device_tables()
+        nrx, ntx = self.num_receive_antennas, self.num_transmit_antennas
+        csi = torch.empty((nrx, ntx, int(num_samples), num_taps), dtype=torch.complex64, device=self.device)
+        _lib.check(_lib.load().hpx_fading_state(_lib.ptr(d["sos"]), self.num_paths, self.num_sinusoids + 1,
+                                                _lib.ptr(d["delays"]), num_taps, _lib.ptr(d["spatial"]), nrx, ntx,
+                                                int(num_samples), _lib.ptr(csi), _lib.stream_handle()))
+        return ChannelStateInformation(ChannelStateFormat.IMPULSE_RESPONSE, csi, num_delay_taps=num_taps)
 
 
 class MultipathFadingRealization:
@@ -193,11 +217,29 @@ class MultipathFadingRealization:
         self.scalars = np.asarray(scalars, dtype=np.float64)
         self.channel = channel
 
-    def sample(self, num_transmit_antennas: int, num_receive_antennas: int, bandwidth: float,
-               device="cuda") -> MultipathFadingSample:
-        """``_sample`` (``fading.py:468-515``) for a link of the given antenna counts and sampling rate."""
+    def sample(self, transmitter, receiver, timestamp: float = 0.0, *args, device="cuda") -> MultipathFadingSample:
+        """``ChannelRealization.sample`` (``channel.py:530-577``): the link from ``transmitter`` to
+        ``receiver`` (``SimulatedDevice`` or ``SimulatedDeviceState``, both of one kind) at
+        ``timestamp``; optional positional carrier frequency and bandwidth default to the
+        transmitter's carrier frequency and sampling rate."""
+
+        if isinstance(transmitter, SimulatedDevice) != isinstance(receiver, SimulatedDevice):
+            raise ValueError("Invalid input argument types for channel sampling.")
+        tx = device_state(transmitter, timestamp)
+        rx = device_state(receiver, timestamp)
+        carrier = float(args[0]) if len(args) > 0 and args[0] is not None else tx.carrier_frequency
+        bandwidth = float(args[1]) if len(args) > 1 and args[1] is not None else tx.sampling_rate
+        sample = self._sample(LinkState(tx, rx, carrier, bandwidth, timestamp), device)
+        for hook in list(self.channel.sample_hooks):
+            hook(sample, tx.device_id, rx.device_id)
+        return sample
+
+    def _sample(self, state: LinkState, device="cuda") -> MultipathFadingSample:
+        """``MultipathFadingRealization._sample`` (``fading.py:468-515``)."""
 
         ch = self.channel
+        num_transmit_antennas = state.transmitter.num_digital_transmit_ports
+        num_receive_antennas = state.receiver.num_digital_receive_ports
         L, N = ch.num_resolvable_paths, ch.num_sinusoids
         if num_receive_antennas > 10 or num_transmit_antennas > 10:
             raise ValueError("the fading spatial response is drawn on a 10 x 10 grid (at most 10 antennas)")
@@ -221,7 +263,21 @@ class MultipathFadingRealization:
         return MultipathFadingSample(ch.power_profile, ch.delays, 2 * np.pi * los_u, -np.pi + 2 * np.pi * nlos_ang_u,
                                      -np.pi + 2 * np.pi * los_ph_u, -np.pi + 2 * np.pi * nlos_ph_u, ch.los_gains,
                                      ch.nlos_gains, ch.los_doppler_frequency, ch.doppler_frequency, spatial, ch.gain,
-                                     bandwidth, device)
+                                     state.bandwidth, device, state.carrier_frequency)
+
+    def reciprocal_sample(self, sample: MultipathFadingSample, transmitter, receiver, carrier_frequency=None,
+                          bandwidth=None) -> MultipathFadingSample:
+        """``_reciprocal_sample`` (``fading.py:517-538``): the same paths over the transposed
+        spatial response (antennas are not re-drawn for the reverse direction)."""
+
+        device_state(transmitter)
+        device_state(receiver)
+        return MultipathFadingSample(sample.power_profile, sample.delay_profile, sample.los_angles,
+                                     sample.nlos_angles, sample.los_phases, sample.nlos_phases, sample.los_gains,
+                                     sample.nlos_gains, sample.los_doppler, sample.nlos_doppler,
+                                     sample.spatial_response.T, sample.gain,
+                                     sample.bandwidth if bandwidth is None else float(bandwidth), sample.device,
+                                     sample.carrier_frequency if carrier_frequency is None else float(carrier_frequency))
 
 
 class MultipathFadingChannel:
@@ -271,6 +327,16 @@ class MultipathFadingChannel:
         self.nlos_gains = np.zeros(self.delays.size)
         self.nlos_gains[~inf] = np.sqrt(1 / ((1 + self.rice_factors[~inf]) * self.num_sinusoids))
         self.realization: MultipathFadingRealization | None = None
+        self.sample_hooks: set = set()
+
+    def add_sample_hook(self, hook) -> None:
+        """Call ``hook(sample, transmitter_id, receiver_id)`` for every sample of this channel's
+        realisations (the sample hooks of ``channel.py:569-575``)."""
+
+        self.sample_hooks.add(hook)
+
+    def remove_sample_hook(self, hook) -> None:
+        self.sample_hooks.discard(hook)
 
     @property
     def seed(self) -> int | None:
@@ -306,12 +372,16 @@ class MultipathFadingChannel:
             self.realization = real
         return real
 
-    def propagate(self, signal, num_receive_antennas: int, bandwidth: float, device="cuda") -> torch.Tensor:
-        """``Channel.propagate`` (``channel.py:910-973``): realise, sample, propagate."""
+    def propagate(self, signal, transmitter, receiver, timestamp: float = 0.0,
+                  interpolation_mode: InterpolationMode = InterpolationMode.NEAREST) -> Signal:
+        """``Channel.propagate`` (``channel.py:910-973``): a new realisation, sampled between the
+        devices at the signal's carrier frequency and sampling rate, propagating ``signal``."""
 
-        x = _as_device_signal(signal, device)
-        sample = self.realize().sample(int(x.shape[0]), num_receive_antennas, bandwidth, device)
-        return sample.propagate(x)
+        if not isinstance(signal, Signal):
+            raise ValueError("Signal is of unsupported type")
+        sample = self.realize().sample(transmitter, receiver, timestamp, signal.carrier_frequency,
+                                       signal.sampling_rate, device=signal.device)
+        return sample.propagate(signal, interpolation_mode)
 
 
 # ---------------------------------------------------------------------- parameterisations

@@ -58,6 +58,31 @@ class LinkConfig:
             if nref and (self.pilots is None or np.shape(self.pilots) != (self.num_tx, nref)):
                 raise ValueError("reference elements need (num_tx, num_references) pilot symbols")
 
+    @classmethod
+    def from_channel(cls, channel, num_rx: int, num_tx: int, num_subcarriers: int, words: int, prefix_ratio: float,
+                     modulation_order: int, bandwidth: float, **kwargs) -> "LinkConfig":
+        """The link of a configured ``MultipathFadingChannel`` (or one of its parameterisations:
+        ``TDL``, ``Cost259``, ``Exponential``) between a transmitter of ``num_tx`` and a receiver of
+        ``num_rx`` antennas, sampled at ``bandwidth``: its sorted delays, powers and Rice factors,
+        sinusoid count, NLOS and LOS doppler_frequency, gain and the covariances its antenna
+        correlation gives the two ends (``fading.py:476-489``; a ``StandardAntennaCorrelation`` treats
+        the transmitter as base station and the receiver as terminal, ``correlation.py:72-106``)."""
+
+        corr = getattr(channel, "antenna_correlation", None)
+        if corr is not None:
+            if hasattr(corr, "covariance_for"):
+                kwargs.setdefault("corr_rx", corr.covariance_for(num_rx, True))
+                kwargs.setdefault("corr_tx", corr.covariance_for(num_tx, False))
+            else:
+                kwargs.setdefault("corr_rx", corr.sample_covariance(num_rx))
+                kwargs.setdefault("corr_tx", corr.sample_covariance(num_tx))
+        kwargs.setdefault("num_sinusoids", int(channel.num_sinusoids))
+        kwargs.setdefault("gain", float(channel.gain))
+        kwargs.setdefault("los_doppler", float(channel.los_doppler_frequency))
+        return cls(num_rx, num_tx, num_subcarriers, words, prefix_ratio, modulation_order, bandwidth,
+                   np.array(channel.delays, dtype=np.float64), np.array(channel.power_profile, dtype=np.float64),
+                   np.array(channel.rice_factors, dtype=np.float64), float(channel.doppler_frequency), **kwargs)
+
     # ---------------------------------------------------------------- derived
     @property
     def prefix(self) -> int:

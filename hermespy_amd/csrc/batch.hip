@@ -590,7 +590,9 @@ int64_t layout_lds(BatchArgs& a, int nr, int nt, bool lean) {
 // Validates the link (and frame, if any) and computes the launch geometry and LDS layout.
 // allow_lean: an 8x8 link beyond one CU's LDS gets the lean layout (heterogeneous batches,
 // which provide its workspace).
-int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out, bool allow_lean = false) {
+// exact_taps: the caller evaluates every tap sample directly (the float64 path), so no Taylor
+// block size bounds the Doppler rate.
+int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out, bool allow_lean = false, bool exact_taps = false) {
   HPX_REQUIRE(l, "hpx_run_batch: null link");
   if (f) {
     HPX_REQUIRE(f->elements && f->word_data, "hpx_run_batch_frames: null frame arrays");
@@ -642,6 +644,7 @@ int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out, bool allow_lean 
   a.tx_scale = sqrt(l->device_power) / sqrt(static_cast<double>(l->num_subcarriers));
   const double max_rate = std::max(fabs(l->doppler), fabs(l->los_doppler)) / l->bandwidth;
   a.log2b = taylor_log2_block(max_rate, a.terms, 11);
+  if (exact_taps) a.log2b = std::max(a.log2b, 6);
   if (a.log2b < 6) {
     set_error("hpx_run_batch: Doppler rate %.3g rad/sample too high for the block Taylor taps", max_rate);
     return HPX_ERR_UNSUPPORTED;
@@ -1178,7 +1181,7 @@ extern "C" int hpx_run_batch_f64(const hpx_link* link_host, const int32_t* delay
     return HPX_ERR_UNSUPPORTED;
   }
   Plan plan;
-  const int st = make_plan(link_host, nullptr, &plan);
+  const int st = make_plan(link_host, nullptr, &plan, false, true);
   if (st != HPX_OK) return st;
   if (num_drops == 0) return HPX_OK;
   BatchArgs& a = plan.args;

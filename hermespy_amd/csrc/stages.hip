@@ -190,6 +190,77 @@ extern "C" int hpx_tdl_propagate(const hpx_c64* x, int num_tx, int64_t ld_x, int
   return check_launch("hpx_tdl_propagate");
 }
 
+// ------------------------------------------------------------------ dense impulse response
+// csi[r][t][n][d] = S[r][t] * sum over paths of delay d of h_p(n); one thread per (n, d).
+__global__ void __launch_bounds__(256) fading_state_kernel(const double* __restrict__ sos, int paths, int terms,
+                                                           const int32_t* __restrict__ delays, int num_taps,
+                                                           const float2* __restrict__ spatial, int nrx, int ntx,
+                                                           int64_t m_len, float2* __restrict__ csi) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m_len * num_taps) return;
+  const int64_t n = i / num_taps;
+  const int d = static_cast<int>(i - n * num_taps);
+  float2 h = make_float2(0.f, 0.f);
+  for (int p = 0; p < paths; ++p)
+    if (delays[p] == d) h = cadd(h, tap_direct(sos + static_cast<int64_t>(p) * terms * 3, terms, n));
+  const int64_t plane = m_len * num_taps;
+  for (int e = 0; e < nrx * ntx; ++e) csi[e * plane + i] = cmul(spatial[e], h);
+}
+
+extern "C" int hpx_fading_state(const double* sos, int num_paths, int num_terms, const int32_t* delays,
+                                int32_t num_taps, const hpx_c64* spatial, int num_rx, int num_tx, int64_t num_samples,
+                                hpx_c64* csi, void* stream) {
+  HPX_REQUIRE(sos && delays && spatial && csi, "hpx_fading_state: null pointer");
+  HPX_REQUIRE(num_paths >= 1 && num_terms >= 1 && num_taps >= 1 && num_samples >= 0 && num_rx >= 1 && num_tx >= 1,
+              "hpx_fading_state: invalid sizes");
+  const int64_t total = num_samples * num_taps;
+  if (total == 0) return HPX_OK;
+  HPX_REQUIRE(total / 256 < (int64_t(1) << 31), "hpx_fading_state: too many samples");
+  hipLaunchKernelGGL(fading_state_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
+                     HPX_STREAM(stream), sos, num_paths, num_terms, delays, num_taps,
+                     reinterpret_cast<const float2*>(spatial), num_rx, num_tx, num_samples,
+                     reinterpret_cast<float2*>(csi));
+  return check_launch("hpx_fading_state");
+}
+
+// y[r][n] = sum_{t, d} csi[r][t][n - d][d] x[t][n - d]; one thread per (r, n).
+__global__ void __launch_bounds__(256) csi_propagate_kernel(const float2* __restrict__ csi, int nrx, int ntx,
+                                                            int64_t csi_samples, int num_taps,
+                                                            const float2* __restrict__ x, int64_t ld_x, int64_t m_len,
+                                                            float2* __restrict__ y, int64_t ld_y) {
+  const int64_t n = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  const int64_t out_len = m_len + num_taps - 1;
+  if (n >= out_len) return;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int t = 0; t < ntx; ++t) {
+    const float2* h = csi + (static_cast<int64_t>(r) * ntx + t) * csi_samples * num_taps;
+    for (int d = 0; d < num_taps; ++d) {
+      const int64_t m = n - d;
+      if (m < 0 || m >= m_len) continue;
+      acc = cfma(h[m * num_taps + d], x[t * ld_x + m], acc);
+    }
+  }
+  y[r * ld_y + n] = acc;
+}
+
+extern "C" int hpx_csi_propagate(const hpx_c64* csi, int num_rx, int num_tx, int64_t csi_samples, int32_t num_taps,
+                                 const hpx_c64* x, int64_t ld_x, int64_t num_samples, hpx_c64* y, int64_t ld_y,
+                                 void* stream) {
+  HPX_REQUIRE(csi && x && y, "hpx_csi_propagate: null pointer");
+  HPX_REQUIRE(num_rx >= 1 && num_rx <= 65535 && num_tx >= 1 && num_taps >= 1 && num_samples >= 0,
+              "hpx_csi_propagate: invalid sizes");
+  HPX_REQUIRE(csi_samples >= num_samples, "hpx_csi_propagate: the channel state covers %lld samples, the signal %lld",
+              static_cast<long long>(csi_samples), static_cast<long long>(num_samples));
+  HPX_REQUIRE(ld_x >= num_samples && ld_y >= num_samples + num_taps - 1, "hpx_csi_propagate: leading dimension too small");
+  const int64_t out_len = num_samples + num_taps - 1;
+  if (num_samples == 0) return HPX_OK;
+  hipLaunchKernelGGL(csi_propagate_kernel, dim3(static_cast<unsigned>((out_len + 255) / 256), num_rx), dim3(256), 0,
+                     HPX_STREAM(stream), reinterpret_cast<const float2*>(csi), num_rx, num_tx, csi_samples, num_taps,
+                     reinterpret_cast<const float2*>(x), ld_x, num_samples, reinterpret_cast<float2*>(y), ld_y);
+  return check_launch("hpx_csi_propagate");
+}
+
 // ------------------------------------------------------------------ AWGN
 __global__ void __launch_bounds__(256) awgn_kernel(float2* __restrict__ sig, int64_t ld, int64_t len, uint64_t seed,
                                                    uint32_t drop, float sigma) {

@@ -116,6 +116,10 @@ class MassiveBatch:
         if n > self.chunk:
             raise ValueError(f"at most {self.chunk} drops per launch")
         c = self.cfg
+        if noise_power.dtype != torch.float64:
+            raise ValueError("noise_power must be float64")
+        if errors.dtype != torch.int32 or errors.numel() < n:
+            raise ValueError("errors must be an int32 tensor with one entry per drop")
         if equalized is not None and (equalized.dtype != torch.complex64 or
                                       equalized.numel() < n * c.num_users * c.words * c.active):
             raise ValueError("equalized must be complex64 [drops][users][words][active]")

@@ -380,3 +380,158 @@ class DropReplay:
 
     def close(self) -> None:
         self.file.close()
+
+
+# ---------------------------------------------------------------- reference realisation records
+# Serialization tags (``Serializable.serialization_tag``: module + class name, factory.py:88-93)
+REALIZATION_TAG = "hermespy.channel.fading.fading.MultipathFadingRealization"
+STATIC_REALIZATION_TAG = "hermespy.channel.consistent.StaticConsistentRealization"
+UNIFORM_TAG = "hermespy.channel.consistent.ConsistentUniform"
+CUSTOM_CORRELATION_TAG = "hermespy.channel.fading.fading.CustomAntennaCorrelation"
+DROP_TAG = "hermespy.simulation.drop.SimulatedDrop"
+
+
+def realization_variables(num_paths: int, num_sinusoids: int) -> list:
+    """(name, shape, offset) of the uniform variables a ``MultipathFadingChannel`` registers with
+    its ``ConsistentGenerator``, in registration order (``fading.py:737-754``; offsets accumulate,
+    ``consistent.py:410-432``): the scalars of a realisation are their concatenation."""
+
+    L, N = int(num_paths), int(num_sinusoids)
+    out, offset = [], 0
+    for name, shape in (("antenna_correlation_variable", (10, 10)), ("los_angles_variable", (L,)),
+                        ("path_angles_variable", (L, N)), ("los_phases_variable", (L,)),
+                        ("path_phases_variable", (L, N))):
+        out.append((name, shape, offset))
+        offset += int(np.prod(shape))
+    return out
+
+
+def write_realization(f: H5File, path: str, normals, link: LinkConfig, doppler=None) -> None:
+    """A ``MultipathFadingRealization`` group at ``path`` in the reference's layout
+    (``fading.py:540-556``): the ``StaticConsistentRealization`` of its scalars
+    (``random_realization/scalar_samples``, ``consistent.py:254-255``), the five
+    ``ConsistentUniform`` variables (``shape`` dataset, ``offset`` attribute, ``consistent.py:93-95``),
+    the profiles and gains as datasets, the Doppler frequencies and ``gain`` as attributes and, for
+    a custom antenna correlation, its ``covariance_matrix``; ``type`` tags are written after the
+    members (``factory.py:940-943``).  doppler: (NLOS, LOS) of this realisation, default the link's."""
+
+    z = np.ascontiguousarray(np.asarray(normals, dtype=np.float64).reshape(-1))
+    if z.size != link.num_normals:
+        raise ValueError(f"a realisation of this link has {link.num_normals} scalars, not {z.size}")
+    nlos, los = (link.doppler, link.doppler if link.los_doppler is None else link.los_doppler) if doppler is None \
+        else (float(doppler[0]), float(doppler[1]))
+    f.create_group(path)
+    f.create_group(f"{path}/random_realization")
+    f.append(f"{path}/random_realization/scalar_samples", z)
+    f.set_attr(f"{path}/random_realization", "type", STATIC_REALIZATION_TAG)
+    for name, shape, offset in realization_variables(link.num_paths, link.num_sinusoids):
+        g = f"{path}/{name}"
+        f.create_group(g)
+        f.append(f"{g}/shape", np.asarray(shape, dtype=np.int64))
+        f.set_attr(g, "offset", int(offset))
+        f.set_attr(g, "type", UNIFORM_TAG)
+    f.append(f"{path}/power_profile", np.asarray(link.power_profile, dtype=np.float64))
+    f.append(f"{path}/delay_profile", np.asarray(link.delays, dtype=np.float64))
+    f.append(f"{path}/los_gains", np.asarray(link.los_gains, dtype=np.float64))
+    f.append(f"{path}/nlos_gains", np.asarray(link.nlos_gains, dtype=np.float64))
+    f.set_attr(path, "los_doppler", float(los))
+    f.set_attr(path, "nlos_doppler", float(nlos))
+    if link.corr_rx is not None:
+        if link.corr_tx is None or not np.array_equal(np.asarray(link.corr_rx)[:link.num_tx, :link.num_tx],
+                                                      np.asarray(link.corr_tx)):
+            raise ValueError("the reference records one custom covariance for both ends of a link")
+        g = f"{path}/antenna_correlation"
+        f.create_group(g)
+        f.append(f"{g}/covariance_matrix", np.asarray(link.corr_rx, dtype=np.complex128))
+        f.set_attr(g, "type", CUSTOM_CORRELATION_TAG)
+    f.set_attr(path, "gain", float(link.gain))
+    f.set_attr(path, "type", REALIZATION_TAG)
+
+
+def read_realization(f: H5File, path: str, link: LinkConfig) -> np.ndarray:
+    """The scalars of the ``MultipathFadingRealization`` group at ``path`` in this library's order
+    (the five variables concatenated), gathered by each variable's recorded ``offset`` and
+    ``shape`` from ``random_realization/scalar_samples``; the recorded profiles and gains must be
+    the link's (ValueError otherwise)."""
+
+    if f.get_str(path, "type") != REALIZATION_TAG:
+        raise ValueError(f"'{path}' is not a MultipathFadingRealization")
+    if f.get_str(f"{path}/random_realization", "type") != STATIC_REALIZATION_TAG:
+        raise ValueError("only static (spatially invariant) realisations replay on this path")
+    scalars = np.asarray(f.read(f"{path}/random_realization/scalar_samples"), dtype=np.float64).reshape(-1)
+    for name, expect in (("power_profile", link.power_profile), ("delay_profile", link.delays),
+                         ("los_gains", link.los_gains), ("nlos_gains", link.nlos_gains)):
+        got = np.asarray(f.read(f"{path}/{name}"), dtype=np.float64)
+        if got.shape != np.shape(expect) or not np.allclose(got, expect, rtol=1e-12, atol=0.0):
+            raise ValueError(f"recorded {name} differs from the link's")
+    parts = []
+    for name, shape, _ in realization_variables(link.num_paths, link.num_sinusoids):
+        g = f"{path}/{name}"
+        rec_shape = tuple(int(v) for v in f.read(f"{g}/shape", dtype=np.int64))
+        offset = int(f.get_attr(g, "offset", np.int64)[0])
+        if rec_shape != tuple(shape):
+            raise ValueError(f"recorded {name} has shape {rec_shape}, the link needs {tuple(shape)}")
+        count = int(np.prod(shape))
+        if offset < 0 or offset + count > scalars.size:
+            raise ValueError(f"recorded {name} lies outside the realisation's scalars")
+        parts.append(scalars[offset:offset + count])
+    return np.concatenate(parts)
+
+
+def _names(f: H5File, path: str) -> list:
+    """Members of a group (none for an empty or missing group)."""
+
+    if not f.exists(path):
+        return []
+    try:
+        return f.list(path)
+    except OSError:
+        return []
+
+
+def write_reference_drops(file: str | os.PathLike, link: LinkConfig, normals, campaign: str | None = None,
+                          overwrite: bool = False, doppler=None, timestamps=None) -> None:
+    """Drops in the reference's recording layout (``Scenario.record`` + ``SimulatedDrop.serialize``,
+    ``scenario.py:368-439,840-851``, ``simulation/drop.py:54-56``, ``core/drop.py:92-95``):
+    ``drops/0000, 0001, ...`` tagged ``SimulatedDrop`` with a ``timestamp`` attribute, empty
+    ``transmissions`` / ``receptions`` sequences (signals are re-simulated here, not stored) and
+    the link's realisation as ``channel_realizations/0000``.  normals: (drops, num_normals)."""
+
+    z = np.asarray(normals, dtype=np.float64).reshape(-1, link.num_normals)
+    mode = TRUNCATE if overwrite else (APPEND if Path(file).exists() else CREATE)
+    with H5File(file, mode) as f:
+        f.set_attr("/", "version", FORMAT_VERSION)
+        f.set_attr("/", "timestamp", time.time())
+        group = "/" if campaign is None else f"/{campaign}"
+        if campaign is not None:
+            f.create_group(group)
+        base = group.rstrip("/")
+        f.create_group(f"{base}/arrays")
+        f.create_group(f"{base}/drops")
+        first = len(_names(f, f"{base}/drops"))
+        for k in range(z.shape[0]):
+            d = f"{base}/drops/{first + k:04d}"
+            f.create_group(d)
+            f.set_attr(d, "timestamp", float(0.0 if timestamps is None else timestamps[k]))
+            f.create_group(f"{d}/transmissions")
+            f.create_group(f"{d}/receptions")
+            f.create_group(f"{d}/channel_realizations")
+            write_realization(f, f"{d}/channel_realizations/0000", z[k], link,
+                              None if doppler is None else doppler[k])
+            f.set_attr(d, "type", DROP_TAG)
+
+
+def read_reference_drops(file: str | os.PathLike, link: LinkConfig, campaign: str | None = None) -> np.ndarray:
+    """(drops, num_normals) realisation scalars of the drops of a reference-layout recording, in
+    drop order (``Scenario.replay`` reads ``drops`` as a sequence, ``scenario.py:486-506``)."""
+
+    with H5File(file, READ) as f:
+        base = "" if campaign is None else f"/{campaign}"
+        names = sorted(_names(f, f"{base}/drops"), key=int)
+        out = np.empty((len(names), link.num_normals), dtype=np.float64)
+        for k, name in enumerate(names):
+            d = f"{base}/drops/{name}"
+            if f.get_str(d, "type") != DROP_TAG:
+                raise ValueError(f"'{d}' is not a SimulatedDrop")
+            out[k] = read_realization(f, f"{d}/channel_realizations/0000", link)
+    return out

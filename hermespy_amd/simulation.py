@@ -53,9 +53,13 @@ class BatchedLink:
             float(link.doppler if link.los_doppler is None else link.los_doppler), float(link.gain),
             float(link.device_power), int(link.workgroup_size))
         self.lds_bytes = int(lib.hpx_run_batch_lds_bytes(ctypes.byref(self.c_link)))
+        self._f32_error = None
         if self.lds_bytes <= 0:
-            # surface the validation message
-            _lib.check(_lib.HPX_ERR_UNSUPPORTED)
+            msg = _lib.last_error()
+            # a single-antenna link beyond the float32 path's Taylor taps may still run in float64
+            if not ("Doppler rate" in msg and link.num_rx == link.num_tx == 1):
+                raise ValueError(msg)
+            self._f32_error = msg
         dev = self.device
         self.delays = torch.as_tensor(link.delay_samples, dtype=torch.int32, device=dev)
         self.path_gains = torch.as_tensor(link.path_gains, dtype=torch.float64, device=dev).contiguous()
@@ -99,6 +103,8 @@ class BatchedLink:
         response S (drops, Nrx, Ntx); the ideal CSI of element (w, k) is S c[w, k]."""
 
         lk = self.link
+        if self._f32_error is not None:
+            raise ValueError(self._f32_error)
         n = int(noise_power.numel())
         if errors.numel() < n or errors.dtype != torch.int32:
             raise ValueError("errors must be an int32 tensor with one entry per drop")
@@ -117,6 +123,12 @@ class BatchedLink:
                 doppler = torch.as_tensor(np.ascontiguousarray(doppler, dtype=np.float64), device=self.device)
             elif doppler.dtype != torch.float64 or doppler.numel() != 2 * n:
                 raise ValueError("per-drop Doppler must be a float64 (num_drops, 2) tensor")
+            else:
+                # the launch's Taylor blocks are sized for the link's Doppler: one host read per launch
+                bound = torch.tensor([lk.doppler, lk.doppler if lk.los_doppler is None else lk.los_doppler],
+                                     dtype=torch.float64, device=doppler.device)
+                if n and bool((doppler.reshape(n, 2).abs() > bound.abs() * (1 + 1e-12)).any()):
+                    raise ValueError("per-drop Doppler must be (num_drops, 2) and bounded by the link's Doppler")
             self._doppler = doppler  # held until the next launch
         if drop_index is not None and (drop_index.dtype != torch.int32 or drop_index.numel() < n
                                        or not drop_index.is_contiguous()):
@@ -305,6 +317,29 @@ class BatchedSimulation:
         per_drops = np.bincount(idx, minlength=snr.size).astype(np.int64)
         return BatchResult(snr, per_err, per_drops, self.link.bits_per_drop)
 
+    def replay_realizations(self, file, campaign: str | None = None, snr_db=None,
+                            drops_per_launch: int = 4096) -> BatchResult:
+        """Drops over the channel realisations of a recording in the reference's layout
+        (``drops/NNNN/channel_realizations/0000``, see :func:`hermespy_amd.record.write_reference_drops`):
+        drop k runs over recorded realisation k (its ``scalar_samples``), with Philox drop k's bits
+        and noise at SNR point k % len(snr_db)."""
+
+        from .record import read_reference_drops
+
+        z = read_reference_drops(file, self.link, campaign)
+        snr = np.asarray(self.link.snr_db if snr_db is None else snr_db, dtype=np.float64)
+        n = z.shape[0]
+        errors = torch.empty(n, dtype=torch.int32, device=self.device)
+        for start in range(0, n, drops_per_launch):
+            cnt = min(drops_per_launch, n - start)
+            normals = torch.as_tensor(np.ascontiguousarray(z[start:start + cnt]), device=self.device)
+            self.batched.run(self.seed, start, self.noise_powers(start, cnt, snr), errors[start:start + cnt],
+                             normals=normals)
+        err = errors.cpu().numpy().astype(np.int64)
+        idx = np.arange(n) % snr.size
+        return BatchResult(snr, np.bincount(idx, weights=err, minlength=snr.size).astype(np.int64),
+                           np.bincount(idx, minlength=snr.size).astype(np.int64), self.link.bits_per_drop)
+
     def _run_replay(self, num_drops, snr, drops_per_launch) -> BatchResult:
         rp = self._replay
         left = rp.num_drops - self._drop_counter
@@ -341,17 +376,21 @@ class ScenarioMode(IntEnum):
 
 
 class SingleDropLink:
-    """The plain GPU path: one drop at a time through the stage entry points.
+    """The plain GPU path: one drop at a time through the stage entry points, written against
+    the reference's typed interfaces.
 
-    Per drop: realisation normals (device) -> host realisation sample (the
-    reference's ``MultipathFadingRealization._sample``) -> bits -> mapping ->
-    OFDM modulation -> TDL propagation -> AWGN -> demodulation -> ideal CSI ->
-    equalisation -> hard demapping -> bit error count, each a separate launch,
-    consuming the same Philox streams as the fused batch kernel.
+    Per drop: realisation normals (device) -> ``realization.sample(transmitter, receiver)``
+    (the reference's ``MultipathFadingRealization._sample``, on the host; the ideal estimator
+    caches the sample through the channel's sample hook) -> bits -> ``map`` / ``place`` /
+    ``modulate`` -> ``sample.propagate(Signal)`` -> AWGN -> ``demodulate`` ->
+    ``estimate_channel(Symbols) -> StatedSymbols`` -> ``pick`` -> ``equalize_channel`` ->
+    ``unmap`` -> ``BitErrorEvaluator.evaluate``, each a separate launch, consuming the same
+    Philox streams as the fused batch kernel.
     """
 
     def __init__(self, link: LinkConfig, seed: int = 20261202, device: str = "cuda") -> None:
         from .channel import CustomAntennaCorrelation, MultipathFadingChannel
+        from .core import SimulatedDevice
         from .modem import (BitErrorEvaluator, GridElement, GridResource, MMSEChannelEqualization, OFDMIdealChannelEstimation,
                             OFDMWaveform, SymbolSection, ZeroForcingChannelEqualization)
 
@@ -368,44 +407,44 @@ class SingleDropLink:
         res = GridResource(1, "CYCLIC", link.prefix_ratio, [GridElement("DATA", link.num_subcarriers)])
         self.waveform = OFDMWaveform([res], [SymbolSection(link.words, [0])], link.num_subcarriers,
                                      link.dc_suppression, link.modulation_order)
-        self.estimator = OFDMIdealChannelEstimation(self.waveform)
-        self.zf = ZeroForcingChannelEqualization()
-        self.mmse = MMSEChannelEqualization()
+        self.transmitter = SimulatedDevice(link.num_tx, sampling_rate=link.bandwidth, power=link.device_power)
+        self.receiver = SimulatedDevice(link.num_rx, sampling_rate=link.bandwidth)
+        self.estimator = OFDMIdealChannelEstimation(self.channel, self.transmitter, self.receiver, self.waveform)
+        self.equalizer = (ZeroForcingChannelEqualization() if link.detector == "zf" else MMSEChannelEqualization())
         self.ber = BitErrorEvaluator()
 
     def run_drop(self, drop: int, snr_db: float, errors: torch.Tensor | None = None,
                  return_equalized: bool = False):
         from .channel import MultipathFadingRealization
+        from .core import Signal
+        from .modem import Symbols
 
         lk, lib, st = self.link, _lib.load(), _lib.stream_handle()
         dev = self.device
         # channel realisation: the drop's normals, sampled on the host like the reference
         z = torch.empty(self.channel.num_scalars, dtype=torch.float64, device=dev)
         _lib.check(lib.hpx_realization_normals(self.seed, drop, z.numel(), _lib.ptr(z), st))
-        sample = MultipathFadingRealization(z.cpu().numpy(), self.channel).sample(lk.num_tx, lk.num_rx, lk.bandwidth, dev)
+        sample = MultipathFadingRealization(z.cpu().numpy(), self.channel).sample(self.transmitter, self.receiver,
+                                                                                  device=dev)
         # transmitter
         nbits = self.waveform.bits_per_frame()
         bits = torch.empty((lk.num_tx, nbits), dtype=torch.uint8, device=dev)
         for t in range(lk.num_tx):
             _lib.check(lib.hpx_stream_bits(self.seed, drop, _lib.STREAM_BITS + t, nbits, _lib.ptr(bits[t]), st))
-        symbols = self.waveform.map(bits.reshape(-1)).reshape(lk.num_tx, -1)
-        x = self.waveform.modulate(self.waveform.place(symbols))
-        if lk.device_power != 1.0:
-            x = x * float(np.sqrt(lk.device_power))
+        symbols = self.waveform.map(bits.reshape(-1)).raw.reshape(lk.num_tx, -1)
+        x = self.waveform.modulate(self.waveform.place(Symbols(symbols)))
+        if self.transmitter.power != 1.0:
+            x = x * float(np.sqrt(self.transmitter.power))  # simulated_device.py:1688-1692
         # channel and noise
-        y = sample.propagate(x)
+        received = sample.propagate(Signal(x, lk.bandwidth))
+        y = received.samples
         p = float(lk.noise_power(snr_db))
         _lib.check(lib.hpx_awgn(_lib.ptr(y), y.shape[0], y.shape[1], y.shape[1], self.seed, drop, p, st))
         # receiver
-        grid = self.waveform.demodulate(y)
-        h = self.estimator.estimate_channel(sample)
-        yd = self.waveform.pick(grid)
-        hd = self.waveform.pick(h)
-        if lk.detector == "zf":
-            xh = self.zf.equalize_channel(yd, hd)
-        else:
-            self.mmse.noise_variance = p
-            xh = self.mmse.equalize_channel(yd, hd)
-        rx_bits = self.waveform.unmap(xh)
-        acc = self.ber.evaluate(bits, rx_bits, errors)
-        return (acc, xh) if return_equalized else acc
+        stated = self.estimator.estimate_channel(self.waveform.demodulate(y), lk.bandwidth, 1)
+        if hasattr(self.equalizer, "noise_variance"):
+            self.equalizer.noise_variance = p
+        equalized = self.equalizer.equalize_channel(self.waveform.pick(stated))
+        rx_bits = self.waveform.unmap(equalized)
+        acc = self.ber.evaluate(bits, rx_bits, errors).errors
+        return (acc, equalized.raw.reshape(lk.num_tx, -1)) if return_equalized else acc

@@ -84,6 +84,20 @@ int hpx_tdl_propagate(const hpx_c64* x, int num_tx, int64_t ld_x, int64_t num_sa
                       int32_t max_delay, const hpx_c64* spatial, int num_rx, hpx_c64* y, int64_t ld_y,
                       void* stream);
 
+/* Dense impulse response csi[r][t][n][d] = S[r][t] sum_{p: delays[p] == d} h_p(n) of a sampled
+ * link, n < num_samples, d < num_taps (paths beyond num_taps - 1 are left out), with h_p the
+ * sum-of-sinusoids taps of hpx_fading_taps (direct evaluation).
+ * Replaces MultipathFadingSample.state, hermespy/channel/fading/fading.py:345-369. */
+int hpx_fading_state(const double* sos, int num_paths, int num_terms, const int32_t* delays, int32_t num_taps,
+                     const hpx_c64* spatial, int num_rx, int num_tx, int64_t num_samples, hpx_c64* csi,
+                     void* stream);
+
+/* Propagation through a dense impulse response csi [num_rx][num_tx][csi_samples][num_taps]:
+ * y[r][n] = sum_{t, d} csi[r][t][n - d][d] x[t][n - d], n < num_samples + num_taps - 1.
+ * Replaces ChannelStateInformation.propagate, hermespy/core/channel.py:493-528. */
+int hpx_csi_propagate(const hpx_c64* csi, int num_rx, int num_tx, int64_t csi_samples, int32_t num_taps,
+                      const hpx_c64* x, int64_t ld_x, int64_t num_samples, hpx_c64* y, int64_t ld_y, void* stream);
+
 /* Adds complex AWGN of total power `power` (sqrt(power/2) per component) to
  * signal[r][n] for n in [0, num_samples), drawing stream HPX_STREAM_NOISE + r of `drop`.
  * Replaces AWGNRealization.add_to, hermespy/simulation/rf/noise/model.py:143-160. */

@@ -49,11 +49,12 @@ class Link:
     device_power: float = 1.0
     frame: ofdm.Frame | None = None  # None: all-data frame of `words` words
     pilots: np.ndarray | None = None  # (num_tx, num_references) reference symbols of each stream
+    los_doppler: float | None = None  # LOS doppler_frequency; None: the NLOS one (fading.py:829-843)
     params: fading.FadingParameters = field(init=False)
 
     def __post_init__(self) -> None:
         self.params = fading.make_parameters(self.delays, self.power_profile, self.rice_factors, self.num_sinusoids,
-                                             self.doppler, None, self.gain, self.corr_rx, self.corr_tx)
+                                             self.doppler, self.los_doppler, self.gain, self.corr_rx, self.corr_tx)
         if self.frame is None:
             self.frame = ofdm.simple_frame(self.num_subcarriers, self.words, self.prefix_ratio, self.dc_suppression)
         if self.frame.num_references and (self.pilots is None or np.shape(self.pilots) !=
@@ -98,8 +99,11 @@ class DropResult:
     csi: np.ndarray | None = None  # (Nrx, Ntx, words, N) ideal CSI
 
 
-def realize(link: Link, seed: int, drop: int) -> fading.FadingSample:
-    z = philox.realization_normals(seed, drop, link.params.num_normals)
+def realize(link: Link, seed: int, drop: int, normals: np.ndarray | None = None) -> fading.FadingSample:
+    """The drop's channel sample: from its Philox realisation normals, or from given normals (the
+    scalars of a recorded ``StaticConsistentRealization``, ``consistent.py:244-260``)."""
+
+    z = philox.realization_normals(seed, drop, link.params.num_normals) if normals is None else normals
     return fading.sample_from_normals(link.params, z, link.num_rx, link.num_tx, link.bandwidth)
 
 
@@ -115,8 +119,8 @@ def transmit(link: Link, seed: int, drop: int):
 
 
 def run_drop(link: Link, seed: int, drop: int, snr_db: float, faithful: bool = True,
-             noise: bool = True) -> DropResult:
-    sample = realize(link, seed, drop)
+             noise: bool = True, normals: np.ndarray | None = None) -> DropResult:
+    sample = realize(link, seed, drop, normals)
     tx_bits, x = transmit(link, seed, drop)
     y = fading.propagate(sample, x)  # (Nrx, M + D)
     p = link.noise_power(snr_db)

@@ -26,7 +26,7 @@ def oracle_link(cfg, detector=None):
     return drop.Link(cfg.num_rx, cfg.num_tx, cfg.num_subcarriers, cfg.words, cfg.prefix_ratio, cfg.modulation_order,
                      cfg.bandwidth, cfg.delays, cfg.power_profile, cfg.rice_factors, cfg.doppler, cfg.num_sinusoids,
                      cfg.corr_rx, cfg.corr_tx, detector or cfg.detector, cfg.dc_suppression, cfg.gain,
-                     cfg.device_power, frame, cfg.pilots)
+                     cfg.device_power, frame, cfg.pilots, cfg.los_doppler)
 
 
 @pytest.fixture(scope="session")

@@ -160,3 +160,36 @@ def test_float64_path_validates_without_a_device(native_lib):
         _lib.check(native_lib.hpx_run_batch_f64(ctypes.byref(small), 0x10, 0x20, 1, 2**32 - 1, 4, 0x30, 0x40, None,
                                                 None))
     assert native_lib.hpx_run_batch_f64(ctypes.byref(small), 0x10, 0x20, 1, 0, 0, 0x30, 0x40, None, None) == 0
+
+
+def test_wrappers_reject_host_tensors(native_lib):
+    """Every array argument of the library is a device pointer: a host tensor, or a view that
+    is not contiguous, is refused with ValueError before any launch."""
+
+    import torch
+
+    from hermespy_amd import _lib
+    from hermespy_amd.configs import siso_64
+    from hermespy_amd.simulation import BatchedLink
+
+    with pytest.raises(ValueError, match="device tensor"):
+        _lib.ptr(torch.zeros(4), "noise_power")
+    link = BatchedLink(siso_64(), device="cpu")
+    with pytest.raises(ValueError, match="device tensor"):
+        link.run(1, 0, torch.zeros(4, dtype=torch.float64), torch.zeros(4, dtype=torch.int32))
+    with pytest.raises(ValueError, match="one entry per drop"):
+        link.run(1, 0, torch.zeros(4, dtype=torch.float64), torch.zeros(2, dtype=torch.int32))
+
+
+def test_massive_run_checks_its_buffers(native_lib):
+    import torch
+
+    from hermespy_amd.massive import MassiveBatch, MassiveConfig
+
+    cfg = MassiveConfig()
+    batch = MassiveBatch.__new__(MassiveBatch)  # buffers are not needed to reach the checks
+    batch.cfg, batch.chunk = cfg, 4
+    with pytest.raises(ValueError, match="float64"):
+        batch.run(1, 0, torch.zeros(2, dtype=torch.float32), torch.zeros(2, dtype=torch.int32))
+    with pytest.raises(ValueError, match="one entry per drop"):
+        batch.run(1, 0, torch.zeros(2, dtype=torch.float64), torch.zeros(1, dtype=torch.int32))

@@ -119,6 +119,22 @@ def test_section_queue_blocks():
     assert q.query_progress()[0] == 1.0
 
 
+def test_section_queue_more_sections_than_drops_per_launch():
+    """With more active sections than drops per launch, every launch fits max_drops and the
+    sections are served round robin, each in consecutive samples."""
+
+    from hermespy_amd.campaign import SectionQueue
+
+    q = SectionQueue(7, 5, 2, 3)
+    seen = {s: [] for s in range(7)}
+    while q.num_active:
+        sec, k = q.next_launch()
+        assert 1 <= sec.size <= 3
+        for s, kk in zip(sec, k):
+            seen[int(s)].append(int(kk))
+    assert all(seen[s] == list(range(5)) for s in range(7))
+
+
 class _FakeLink:
     """Per-drop errors from a fixed function of the Philox drop index (no GPU)."""
 

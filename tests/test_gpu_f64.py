@@ -72,6 +72,32 @@ def test_f64_rician_wide_link_matches_oracle(cuda):
         assert int(err[i]) == ref.errors, (i, int(err[i]), ref.errors)
 
 
+def test_f64_beyond_the_float32_doppler_limit(cuda):
+    """2 kHz Doppler at 1 MHz sampling: beyond the float32 path's Taylor taps (which refuse the
+    link), the float64 path evaluates every tap exactly and matches the float64 oracle."""
+
+    from hermespy_amd.configs import siso_64
+    from hermespy_amd.simulation import BatchedLink
+
+    cfg = dataclasses.replace(siso_64(), doppler=2 * np.pi * 2000.0)
+    print("float64 siso-64 at 2 kHz Doppler vs oracle", flush=True)
+    bl = BatchedLink(cfg)
+    snrs = np.array([5.0, 15.0, 25.0])
+    npw = torch.as_tensor(cfg.noise_power(snrs), dtype=torch.float64, device=cuda)
+    err = torch.empty(snrs.size, dtype=torch.int32, device=cuda)
+    eq = torch.empty((snrs.size, cfg.words * cfg.num_subcarriers), dtype=torch.complex128, device=cuda)
+    with pytest.raises(ValueError, match="Doppler rate"):
+        bl.run(SEED, 3, npw, err)
+    bl.run_f64(SEED, 3, npw, err, eq)
+    torch.cuda.synchronize()
+    err, eq = err.cpu().numpy(), eq.cpu().numpy()
+    link = oracle_link(cfg)
+    for i, snr in enumerate(snrs):
+        ref = odrop.run_drop(link, SEED, 3 + i, float(snr), faithful=True)
+        assert rel_err(eq[i], ref.equalized[0]) < 1e-11, (i, snr)
+        assert int(err[i]) == ref.errors, (i, int(err[i]), ref.errors)
+
+
 def test_f64_rejects_other_links(cuda):
     from hermespy_amd.configs import mimo2x2_512_pilots
     from hermespy_amd.simulation import BatchedLink

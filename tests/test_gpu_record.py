@@ -8,6 +8,8 @@ import torch
 
 pytestmark = pytest.mark.gpu
 
+SEED = 20261202
+
 
 @pytest.fixture(scope="module", autouse=True)
 def h5lib():
@@ -79,3 +81,40 @@ def test_replay_requires_a_file(cuda):
     sim = BatchedSimulation(siso_64())
     with pytest.raises(ValueError, match="valid file path"):
         sim.replay()
+
+
+def test_reference_realizations_replay_against_oracle(cuda, tmp_path):
+    """Channel realisations recorded in the reference's layout (normals drawn by NumPy, not by the
+    library's Philox stream) replay through ``hpx_run_batch_drops`` and match the float64 oracle
+    sampling the same normals (``fading.sample_from_normals``) drop by drop."""
+
+    from conftest import oracle_link
+    from oracle_pool import oracle_drops
+    from parity import check_drop
+
+    from hermespy_amd import record
+    from hermespy_amd.configs import mimo4x4_1024
+    from hermespy_amd.simulation import BatchedLink, BatchedSimulation
+
+    cfg = mimo4x4_1024()
+    snrs = np.array([0.0, 12.0, 24.0, 30.0])
+    z = np.random.default_rng(11).standard_normal((snrs.size, cfg.num_normals))
+    path = tmp_path / "reference_layout.h5"
+    record.write_reference_drops(path, cfg, z, campaign="replay")
+    back = record.read_reference_drops(path, cfg, campaign="replay")
+    assert np.array_equal(back, z)
+    print("reference-layout realisations through hpx_run_batch_drops", flush=True)
+    n = snrs.size
+    npw = torch.as_tensor(cfg.noise_power(snrs), dtype=torch.float64, device="cuda")
+    err = torch.empty(n, dtype=torch.int32, device="cuda")
+    eq = torch.empty((n, cfg.num_tx, cfg.num_data), dtype=torch.complex64, device="cuda")
+    BatchedLink(cfg).run(SEED, 0, npw, err, eq, normals=torch.as_tensor(back, device="cuda"))
+    torch.cuda.synchronize()
+    err, eq = err.cpu().numpy(), eq.cpu().numpy()
+    ora = oracle_drops(oracle_link(cfg), SEED, range(n), snrs, normals=back)
+    for i in range(n):
+        check_drop(eq[i], err[i], ora[i], cfg.modulation_order, 1e-4, ("recorded realisation", i))
+    # the simulation's replay of the file runs the same drops
+    sim = BatchedSimulation(cfg, SEED)
+    res = sim.replay_realizations(path, "replay", snr_db=snrs)
+    assert res.errors.tolist() == err.astype(np.int64).tolist()

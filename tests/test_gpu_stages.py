@@ -191,13 +191,14 @@ def test_channel_mirror_realization_matches_reference_draw_order(cuda):
     """A seeded channel draws los_angle, then the 100 + 2L + 2LN normals per realisation."""
 
     from hermespy_amd.channel import MultipathFadingChannel
+    from hermespy_amd.core import SimulatedDevice
 
     ch = MultipathFadingChannel([0.0, 1e-6], [0.5, 0.5], [0.0, 1.0], num_sinusoids=5, doppler_frequency=10.0, seed=42)
     rng = np.random.default_rng(42)
     assert ch.los_angle == rng.uniform(-np.pi, np.pi)
     real = ch.realize()
     assert np.array_equal(real.scalars, rng.standard_normal(100 + 4 + 20))
-    s = real.sample(2, 2, 1e6)
+    s = real.sample(SimulatedDevice(2, sampling_rate=1e6), SimulatedDevice(2, sampling_rate=1e6))
     params = fading.make_parameters([0.0, 1e-6], [0.5, 0.5], [0.0, 1.0], 5, 10.0)
     os_ = fading.sample_from_normals(params, real.scalars, 2, 2, 1e6)
     np.testing.assert_array_equal(s.nlos_angles, os_.nlos_angles)

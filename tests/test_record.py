@@ -137,3 +137,90 @@ def test_store_library_exports_every_declared_symbol():
     assert len(names) == 11
     for name in names:
         assert hasattr(lib, name), name
+
+
+def test_reference_realization_layout_round_trip(tmp_path):
+    """Drops written in the reference's recording layout (SimulatedDrop groups holding a
+    MultipathFadingRealization, ``fading.py:540-556``, ``consistent.py:93-95,254-255``) read back
+    to the same realisation scalars, and the mirror samples the same channel from them."""
+
+    from hermespy_amd.channel import CustomAntennaCorrelation, MultipathFadingChannel, MultipathFadingRealization
+    from hermespy_amd.core import SimulatedDevice
+
+    cfg = CONFIGS["mimo4x4-1024"]()
+    rng = np.random.default_rng(5)
+    z = rng.standard_normal((3, cfg.num_normals))
+    path = tmp_path / "ref.h5"
+    record.write_reference_drops(path, cfg, z, campaign="c1")
+    record.write_reference_drops(path, cfg, z[:1], campaign="c1")  # appends drop 0003
+    back = record.read_reference_drops(path, cfg, campaign="c1")
+    assert np.array_equal(back, np.concatenate([z, z[:1]]))
+    with record.H5File(path) as f:
+        assert f.list("/c1/drops") == ["0000", "0001", "0002", "0003"]
+        d = "/c1/drops/0001"
+        assert f.get_str(d, "type") == "hermespy.simulation.drop.SimulatedDrop"
+        r = f"{d}/channel_realizations/0000"
+        assert f.get_str(r, "type") == "hermespy.channel.fading.fading.MultipathFadingRealization"
+        assert f.get_str(f"{r}/random_realization", "type") == "hermespy.channel.consistent.StaticConsistentRealization"
+        assert np.array_equal(f.read(f"{r}/random_realization/scalar_samples"), z[1])
+        L, N = cfg.num_paths, cfg.num_sinusoids
+        for name, shape, offset in [("antenna_correlation_variable", (10, 10), 0), ("los_angles_variable", (L,), 100),
+                                    ("path_angles_variable", (L, N), 100 + L),
+                                    ("los_phases_variable", (L,), 100 + L + L * N),
+                                    ("path_phases_variable", (L, N), 100 + 2 * L + L * N)]:
+            assert f.get_str(f"{r}/{name}", "type") == "hermespy.channel.consistent.ConsistentUniform"
+            assert tuple(f.read(f"{r}/{name}/shape")) == shape
+            assert int(f.get_attr(f"{r}/{name}", "offset", np.int64)[0]) == offset
+        assert np.allclose(f.read(f"{r}/power_profile"), cfg.power_profile)
+        assert float(f.get_attr(r, "nlos_doppler")[0]) == cfg.doppler
+        assert np.allclose(f.read(f"{r}/antenna_correlation/covariance_matrix"), cfg.corr_rx)
+    # one realisation read on its own
+    with record.H5File(path, record.APPEND) as f:
+        r = "/c1/drops/0002/channel_realizations/0000"
+        assert np.array_equal(record.read_realization(f, r, cfg), z[2])
+    # the mirror's sample of the replayed scalars is the original realisation's
+    ch = MultipathFadingChannel(cfg.delays, cfg.power_profile, cfg.rice_factors, doppler_frequency=cfg.doppler,
+                                antenna_correlation=CustomAntennaCorrelation(cfg.corr_rx), seed=1)
+    tx, rx = SimulatedDevice(4, sampling_rate=cfg.bandwidth), SimulatedDevice(4, sampling_rate=cfg.bandwidth)
+    a = MultipathFadingRealization(z[1], ch).sample(tx, rx, device="cpu")
+    b = MultipathFadingRealization(back[1], ch).sample(tx, rx, device="cpu")
+    assert np.array_equal(a.spatial_response, b.spatial_response) and np.array_equal(a.nlos_angles, b.nlos_angles)
+    # a link with other profiles is refused
+    other = CONFIGS["siso-64"]()
+    with pytest.raises(ValueError):
+        record.read_reference_drops(path, other, campaign="c1")
+
+
+def test_reference_realization_offsets_are_honoured(tmp_path):
+    """A realisation whose scalars list the variables in another order (offsets permuted) reads
+    back in this library's order."""
+
+    cfg = CONFIGS["siso-64"]()
+    rng = np.random.default_rng(2)
+    z = rng.standard_normal(cfg.num_normals)
+    path = tmp_path / "perm.h5"
+    with record.H5File(path, record.CREATE) as f:
+        record.write_realization(f, "/r", z, cfg)
+    # rewrite: LOS angles stored last, everything after them moved forward
+    vars_ = record.realization_variables(cfg.num_paths, cfg.num_sinusoids)
+    parts = {name: z[off:off + int(np.prod(shape))] for name, shape, off in vars_}
+    order = ["antenna_correlation_variable", "path_angles_variable", "los_phases_variable", "path_phases_variable",
+             "los_angles_variable"]
+    scalars = np.concatenate([parts[n] for n in order])
+    offsets = np.concatenate([[0], np.cumsum([parts[n].size for n in order])[:-1]])
+    path2 = tmp_path / "perm2.h5"
+    with record.H5File(path2, record.CREATE) as f:
+        f.create_group("/r")
+        f.create_group("/r/random_realization")
+        f.append("/r/random_realization/scalar_samples", scalars)
+        f.set_attr("/r/random_realization", "type", record.STATIC_REALIZATION_TAG)
+        for (name, shape, _), off in zip(sorted(vars_, key=lambda v: order.index(v[0])), offsets):
+            f.create_group(f"/r/{name}")
+            f.append(f"/r/{name}/shape", np.asarray(shape, dtype=np.int64))
+            f.set_attr(f"/r/{name}", "offset", int(off))
+            f.set_attr(f"/r/{name}", "type", record.UNIFORM_TAG)
+        for name, v in (("power_profile", cfg.power_profile), ("delay_profile", cfg.delays),
+                        ("los_gains", cfg.los_gains), ("nlos_gains", cfg.nlos_gains)):
+            f.append(f"/r/{name}", np.asarray(v, dtype=np.float64))
+        f.set_attr("/r", "type", record.REALIZATION_TAG)
+        assert np.array_equal(record.read_realization(f, "/r", cfg), z)
