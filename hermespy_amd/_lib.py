@@ -125,6 +125,7 @@ SIGNATURES = {
     "hpx_demap_hard": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp]),
     "hpx_constellation_host": (c_int, [c_int, c_int, c_vp]),
     "hpx_demap_hard_host": (c_int, [c_vp, c_i64, c_int, c_int, c_vp]),
+    "hpx_demap_hard_f32_host": (c_int, [c_vp, c_i64, c_int, c_vp]),
     "hpx_ofdm_modulate": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_i64, c_vp]),
     "hpx_ofdm_demodulate": (c_int, [c_vp, c_int, c_i64, c_int, c_int, c_int, c_int, c_vp, c_vp]),
     "hpx_ideal_csi": (c_int, [c_vp, c_int, c_int, c_vp, c_i32, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_vp,

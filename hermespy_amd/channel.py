@@ -142,7 +142,8 @@ class MultipathFadingSample:
             self._dev_cache = {
                 "sos": torch.as_tensor(tab, device=self.device).contiguous(),
                 "delays": torch.as_tensor(self.delay_samples, device=self.device),
-                "spatial": torch.as_tensor(self.spatial_response.astype(np.complex64), device=self.device),
+                "spatial": torch.as_tensor(np.ascontiguousarray(self.spatial_response, dtype=np.complex64),
+                                           device=self.device),
                 "max_rate": float(np.max(np.abs(tab[:, :, 0]))) if tab.size else 0.0,
             }
         return self._dev_cache

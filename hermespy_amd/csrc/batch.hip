@@ -58,6 +58,8 @@ struct BatchArgs {
   int64_t frame_len;
   double bandwidth, doppler, los_doppler, tx_scale;
   Modulation mod;
+  DemapBounds dbounds;  // float32 decision boundaries of mod (exact hard decisions)
+  int pad_shift;        // FFT intermediate layout: kFftPadShift, or kFftNoPad where LDS is short
   // arrays
   const int32_t* delays;
   const double* path_gains;
@@ -82,9 +84,12 @@ struct BatchArgs {
   int64_t first_drop;
   // LDS byte offsets
   int off_x, off_coef, off_abar, off_tw, off_bits, off_cons, off_s, off_det, off_red, off_dly, off_tmp_sos, off_tmp_s64;
+  int off_dmb;     // the demapper's boundaries
+  int off_chain;   // per path: next path of the same delay (+1, 0: none) | 1 << 16 for the first one
   // lean drops (working set beyond one CU's LDS): Taylor table and twiddles in device memory
   float2* ws_coef;
-  float2* ws_tw;
+  float2* ws_tw;    // natural twiddles (the lean ideal-CSI sum)
+  float2* ws_ptab;  // FFT pass table
   int off_tmp_dir;  // prologue only: per-term word-mean weights
 };
 
@@ -110,12 +115,15 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
   // a lean drop keeps the Taylor table and the twiddles in its device workspace
   float2* coef = LEAN ? a.ws_coef : reinterpret_cast<float2*>(lds + a.off_coef);
   float2* abar = reinterpret_cast<float2*>(lds + a.off_abar);
-  float2* tw = LEAN ? a.ws_tw : reinterpret_cast<float2*>(lds + a.off_tw);
+  float2* ptab = LEAN ? a.ws_ptab : reinterpret_cast<float2*>(lds + a.off_tw);  // FFT pass table
+  float2* tw = a.ws_tw;  // lean drops only: natural twiddles of the per-subcarrier ideal CSI
   uint32_t* bits = reinterpret_cast<uint32_t*>(lds + a.off_bits);
   float2* cons = reinterpret_cast<float2*>(lds + a.off_cons);
+  DemapBounds* dmb = reinterpret_cast<DemapBounds*>(lds + a.off_dmb);
   float2* smat = reinterpret_cast<float2*>(lds + a.off_s);
   int* red = reinterpret_cast<int*>(lds + a.off_red);
   int* dly = reinterpret_cast<int*>(lds + a.off_dly);  // path delays: LDS broadcasts in the channel loop
+  int* chain = reinterpret_cast<int*>(lds + a.off_chain);  // paths sharing a delay (ideal-CSI row)
   constexpr int kRows = NT > NR ? NT : NR;
   float2* csirow = xbuf + kRows * a.xs + a.x0;  // [N] per-word CSI over the taps (not for lean drops)
   double* uni = reinterpret_cast<double*>(lds + a.off_x);        // prologue only
@@ -150,6 +158,17 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
   // taps (Taylor coefficients and word means), so the channel loop needs no per-path check
   for (int p = tid; p <= L; p += nthreads) dly[p] = p < L && path_in_range(a, p) ? a.delays[p] : 0;
   __syncthreads();
+  // chains of paths sharing a delay, in path order: the CSI row sums their word means in the
+  // order the paths are listed
+  for (int p = tid; p < L; p += nthreads) {
+    const int d = dly[p];
+    bool first = true;
+    for (int q = 0; q < p; ++q) first = first && dly[q] != d;
+    int next = -1;
+    for (int q = L - 1; q > p; --q)
+      if (dly[q] == d) next = q;
+    chain[p] = (next + 1) | (first && d < a.keep_below ? 1 << 16 : 0);
+  }
 
   // spatial response S = R_rx exp(2 pi j U)[:NR, :NT] R_tx (fading.py:476-489)
   for (int e = tid; e < NR * NT; e += nthreads) {
@@ -204,10 +223,20 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     const double2 p = constellation_point(a.order, 1, bits_to_label(static_cast<uint32_t>(v), a.bps));
     cons[v] = make_float2(static_cast<float>(p.x * a.tx_scale), static_cast<float>(p.y * a.tx_scale));
   }
-  for (int k = tid; k < n; k += nthreads) {
-    double s, c;
-    sincospi(-2.0 * static_cast<double>(k) / static_cast<double>(n), &s, &c);
-    tw[k] = make_float2(static_cast<float>(c), static_cast<float>(s));
+  fill_fft_pass_table(ptab, a.log2n);
+  if (tid == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      dmb->a[i] = a.dbounds.a[i];
+      dmb->b[i] = a.dbounds.b[i];
+    }
+  }
+  if (LEAN) {
+    for (int k = tid; k < n; k += nthreads) {
+      double s, c;
+      sincospi(-2.0 * static_cast<double>(k) / static_cast<double>(n), &s, &c);
+      tw[k] = make_float2(static_cast<float>(c), static_cast<float>(s));
+    }
   }
   __syncthreads();
 
@@ -248,11 +277,9 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
   }
   for (int e = tid; e < L * a.words; e += nthreads) {
     const int p = e / a.words, w = e - p * a.words;
-    const double2 m = path_in_range(a, p)
-                          ? tap_mean_weighted(sos + 3 * p * a.terms, amp_dir + p * a.terms, a.terms,
-                                              static_cast<double>(w) * (n + a.cp) + a.cp, n)
-                          : make_double2(0.0, 0.0);
-    abar[e] = make_float2(static_cast<float>(m.x), static_cast<float>(m.y));
+    abar[e] = path_in_range(a, p) ? tap_mean_weighted(sos + 3 * p * a.terms, amp_dir + p * a.terms, a.terms,
+                                                      static_cast<double>(w) * (n + a.cp) + a.cp, n)
+                                  : make_float2(0.f, 0.f);
   }
   __syncthreads();
   // the signal buffer aliased the prologue tables; the frame starts after silence
@@ -268,6 +295,11 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
 
   // ---------------------------------------------------------------- words
   for (int w = 0; w < a.words; ++w) {
+    // The thread index passes through an opaque copy per word: the per-thread index arithmetic
+    // of the word's loops is redone every word instead of being hoisted out of this loop into
+    // registers held across it (which made the channel loop reload spilled values).
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     // bits of this word's data symbols for every stream (stream HPX_STREAM_BITS + t):
     // LSB-first words fw .. fw + nw - 1 of the stream; the word's first bit sits at bit bo
     const int wd0 = a.word_data ? a.word_data[w] : w * n;
@@ -314,18 +346,24 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
       if (!LEAN) csirow[q] = make_float2(0.f, 0.f);
     }
     __syncthreads();
-    lds_fft<+1, FFTV>(xbuf + x0, NT, a.xs, a.log2n, tw);
-    // cyclic prefix; word means of the taps at their delays (paths sharing a delay add up,
-    // taps beyond min(num_taps, N) are cut as fft(n=N) cuts them)
-    // (only the last cpl = min(CP, max delay) prefix samples are ever read by the delayed paths)
+    lds_fft_padded<+1, FFTV>(xbuf + x0, NT, a.xs, a.log2n, ptab, a.pad_shift);
+    // cyclic prefix (only the last cpl = min(CP, max delay) prefix samples are ever read by the
+    // delayed paths)
     for (int e = tid; e < NT * a.cpl; e += nthreads) {
       const int t = e / a.cpl, i = e - t * a.cpl;
       xbuf[t * a.xs + x0 - a.cpl + i] = xbuf[t * a.xs + x0 + n - a.cpl + i];
     }
-    if (!LEAN && tid == 0) {
-      for (int p = 0; p < L; ++p) {
-        const int d = dly[p];
-        if (d >= 0 && d < a.keep_below) csirow[d] = cadd(csirow[d], abar[p * a.words + w]);
+    // CSI row: word means of the taps at their delays, one thread per first path of a delay
+    // (paths sharing a delay add up, taps beyond min(num_taps, N) are cut as fft(n=N) cuts them;
+    // out-of-range paths have delay 0 and zero means)
+    if (!LEAN) {
+      for (int p = tid; p < L; p += nthreads) {
+        int link = chain[p];
+        if (link & (1 << 16)) {
+          float2 c = abar[p * a.words + w];
+          for (int q = (link & 0xffff) - 1; q >= 0; q = (chain[q] & 0xffff) - 1) c = cadd(c, abar[q * a.words + w]);
+          csirow[dly[p]] = c;
+        }
       }
     }
     __syncthreads();
@@ -354,11 +392,11 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
             const int b0 = m >> a.log2b;
             // (24-bit multiply: full rate, where a 32-bit one takes four passes)
             const float2* c0 = coef + __umul24(static_cast<unsigned>(p * a.nb + b0), kTaylorCoeffs);
-            const float s0 = tb.offset(m);
+            const float s0 = tb.offset32(m);
             const float2 h0 = taylor_eval(c0, s0);
             // the pair's second sample opens the next block when m ends one
             const bool wrap = ((m + 1) >> a.log2b) != b0;
-            const float2 h1 = wrap ? taylor_eval(c0 + kTaylorCoeffs, tb.offset(m + 1)) : taylor_eval(c0, s0 + step);
+            const float2 h1 = wrap ? taylor_eval(c0 + kTaylorCoeffs, tb.offset32(m + 1)) : taylor_eval(c0, s0 + step);
             const float2* xc = xbuf + x0 + nl - d;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
@@ -441,7 +479,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     }
     __syncthreads();
     // receive FFT of every antenna and, in the same passes, the CSI row's FFT over the taps
-    lds_fft<-1, FFTV>(xbuf + x0, kRows + (LEAN ? 0 : 1), a.xs, a.log2n, tw);
+    lds_fft_padded<-1, FFTV>(xbuf + x0, kRows + (LEAN ? 0 : 1), a.xs, a.log2n, ptab, a.pad_shift);
 
     // ideal CSI, detection, demapping and bit errors per subcarrier
     for (int q = tid; q < n; q += nthreads) {
@@ -486,7 +524,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
         for (int r = 0; r < NR; ++r) acc = cfma(daa[i * NR + r], yv[r], acc);
         const float den = fmaf(c2, dla[i], s2);
         // rx_scale folds the receiver's 1/sqrt(N); a zero gain realises pinv's dropped singular values
-        const float g = (dla[i] > 0.0f || s2 > 0.0f) && den > 0.0f ? rx_scale / den : 0.0f;
+        const float g = (dla[i] > 0.0f || s2 > 0.0f) && den > 0.0f ? rx_scale * __builtin_amdgcn_rcpf(den) : 0.0f;
         u[i] = cscale(cmulc(c, acc), g);
       }
       float2 xh[NT];
@@ -499,7 +537,7 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
       }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const uint32_t rx = demap_symbol(a.mod, static_cast<double>(xh[t].x), static_cast<double>(xh[t].y));
+        const uint32_t rx = demap_symbol_f32(a.mod, dmb, xh[t].x, xh[t].y);
         const uint32_t tx = symbol_bits(bits + t * a.bits_words, bo + el * a.bps, a.bps);
         errors += __popc(tx ^ rx);
         if (a.equalized) a.equalized[(local_drop * NT + t) * static_cast<int64_t>(a.data_total) + wd0 + el] = xh[t];
@@ -549,9 +587,14 @@ struct Plan {
 
 constexpr int64_t kMaxLds = 160 * 1024;
 
-// LDS layout of a plan; a lean layout leaves out the Taylor table, the twiddles and the CSI row.
-int64_t layout_lds(BatchArgs& a, int nr, int nt, bool lean) {
+// LDS layout of a plan; a lean layout leaves out the Taylor table, the FFT pass table and the
+// CSI row.  pad: rows long enough for the FFT's padded intermediate layout (N + N / 8 after the
+// first data sample).
+int64_t layout_lds(BatchArgs& a, int nr, int nt, bool lean, bool pad) {
   const int rows = std::max(nr, nt) + (lean ? 0 : 1);  // signal rows (+ the CSI row)
+  a.pad_shift = pad ? kFftPadShift : kFftNoPad;
+  a.xs = a.x0 + a.n + (pad ? (a.n >> kFftPadShift) : 0);
+  a.xs += a.xs & 1;
   int64_t o = 0;
   a.off_x = 0;
   const int64_t xbytes = static_cast<int64_t>(rows) * a.xs * sizeof(float2);
@@ -584,6 +627,10 @@ int64_t layout_lds(BatchArgs& a, int nr, int nt, bool lean) {
   o += 16;
   a.off_dly = static_cast<int>(o);
   o += align16(static_cast<int64_t>(a.paths + 1) * sizeof(int32_t));  // + a pad entry for the prefetch
+  a.off_dmb = static_cast<int>(o);
+  o += align16(sizeof(DemapBounds));
+  a.off_chain = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.paths) * sizeof(int32_t));
   return o;
 }
 
@@ -631,6 +678,7 @@ int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out, bool allow_lean 
   a.bps = mod.bps;
   a.order = l->modulation_order;
   a.mod = mod;
+  a.dbounds = demap_bounds(mod);
   a.detector = l->detector;
   a.dc = l->dc_suppression ? 1 : 0;
   a.paths = l->num_paths;
@@ -661,11 +709,10 @@ int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out, bool allow_lean 
     set_error("hpx_run_batch: delay spread of %d samples exceeds one OFDM word", l->max_delay);
     return HPX_ERR_UNSUPPORTED;
   }
-  // LDS row: [previous word's last dx samples | last min(CP, D) prefix samples | N data samples]
+  // LDS row: [previous word's last dx samples | last min(CP, D) prefix samples | N data samples
+  // (| the FFT's padding)]; the row stride a.xs is set with the layout
   a.cpl = std::min(a.cp, l->max_delay);
   a.x0 = a.dx + a.cpl;  // == max delay
-  a.xs = a.x0 + a.n;
-  a.xs += a.xs & 1;
   a.keep_below = static_cast<int>(std::min<int64_t>(std::min<int64_t>(1 + l->max_delay, a.frame_len), a.n));
 
   const int nt = l->num_tx, nr = l->num_rx;
@@ -695,12 +742,28 @@ int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out, bool allow_lean 
   out->threads = threads;
   out->pairs = pairs;
 
-  // LDS layout; the prologue tables (uniforms, SoS table) alias the signal buffer
-  int64_t o = layout_lds(a, nr, nt, false);
+  // LDS layout; the prologue tables (uniforms, SoS table) alias the signal buffer.  The FFT's
+  // padded layout is taken where it keeps as many drops per CU as the plain one.
+  auto per_cu = [&](int64_t bytes) {
+    const int64_t by_waves = static_cast<int64_t>(threads >= 512 ? (nt <= 4 ? 4 : 2) : 4) * 4 * 64 / threads;
+    return bytes > kMaxLds ? int64_t(0) : std::min<int64_t>(kMaxLds / bytes, std::max<int64_t>(1, by_waves));
+  };
+  auto best_layout = [&](bool lean) {
+    BatchArgs padded = a;
+    const int64_t op = layout_lds(padded, nr, nt, lean, true);
+    const int64_t on = layout_lds(a, nr, nt, lean, false);
+    if (per_cu(op) >= per_cu(on) && op <= kMaxLds) {
+      a = padded;
+      return op;
+    }
+    return on;
+  };
+  int64_t o = best_layout(false);
   if (o > kMaxLds && allow_lean && nt == 8) {
-    o = layout_lds(a, nr, nt, true);
+    o = best_layout(true);
     out->lean = true;
-    out->ws_elems = static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs + a.n;
+    // [Taylor table | natural twiddles | FFT pass table]
+    out->ws_elems = static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs + a.n + fft_pass_table_size(a.n);
   }
   out->lds = o;
   if (o > kMaxLds) {
@@ -795,10 +858,14 @@ extern "C" int64_t hpx_uneven_workspace_elements(const hpx_link* link_host) {
 
 extern "C" int64_t hpx_uneven_plan_bytes(void) { return static_cast<int64_t>(sizeof(BatchArgs)); }
 
-extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_host,
-                                  const uint32_t* drop_index_host, int64_t num_drops, uint64_t seed,
-                                  const double* noise_power, int32_t* errors, void* plans_host, int64_t* order_host,
-                                  hpx_uneven_class* classes_host, int max_classes, int* num_classes_host) {
+namespace {
+
+// Plans of a heterogeneous batch.  Lean drops take their workspace from tables_host[i].workspace,
+// or, with scratch_base set, consecutive slices of scratch_base (scratch_elems complex64 values).
+int uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_host, const uint32_t* drop_index_host,
+                   int64_t num_drops, uint64_t seed, const double* noise_power, int32_t* errors, void* plans_host,
+                   int64_t* order_host, hpx_uneven_class* classes_host, int max_classes, int* num_classes_host,
+                   float2* scratch_base, int64_t scratch_elems) {
   HPX_REQUIRE(num_drops >= 0 && num_drops <= 0x7fffffff && max_classes >= 1, "hpx_uneven_prepare: invalid sizes");
   HPX_REQUIRE(num_classes_host, "hpx_uneven_prepare: null pointer");
   *num_classes_host = 0;
@@ -807,6 +874,7 @@ extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tab
                   classes_host,
               "hpx_uneven_prepare: null pointer");
   std::vector<Plan> plans(static_cast<size_t>(num_drops));
+  int64_t scratch_used = 0;
   for (int64_t i = 0; i < num_drops; ++i) {
     const hpx_drop_tables& tb = tables_host[i];
     HPX_REQUIRE(tb.delays && tb.path_gains, "hpx_uneven_prepare: drop %lld has no path tables",
@@ -817,10 +885,22 @@ extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tab
     if (st != HPX_OK) return st;
     BatchArgs& a = plans[i].args;
     if (plans[i].lean) {
-      HPX_REQUIRE(tb.workspace, "hpx_uneven_prepare: drop %lld needs a workspace of %lld elements",
-                  static_cast<long long>(i), static_cast<long long>(plans[i].ws_elems));
-      a.ws_coef = reinterpret_cast<float2*>(tb.workspace);
-      a.ws_tw = reinterpret_cast<float2*>(tb.workspace) + (plans[i].ws_elems - a.n);
+      float2* ws = reinterpret_cast<float2*>(tb.workspace);
+      if (scratch_base) {
+        if (scratch_used + plans[i].ws_elems > scratch_elems) {
+          set_error("hpx_uneven_prepare: the lean drops need more than the %lld complex64 values of the "
+                    "library's scratch array",
+                    static_cast<long long>(scratch_elems));
+          return HPX_ERR_INVALID;
+        }
+        ws = scratch_base + scratch_used;
+        scratch_used += plans[i].ws_elems;
+      }
+      HPX_REQUIRE(ws, "hpx_uneven_prepare: drop %lld needs a workspace of %lld elements", static_cast<long long>(i),
+                  static_cast<long long>(plans[i].ws_elems));
+      a.ws_coef = ws;
+      a.ws_tw = ws + static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs;
+      a.ws_ptab = a.ws_tw + a.n;
     }
     a.delays = tb.delays;
     a.path_gains = tb.path_gains;
@@ -879,6 +959,17 @@ extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tab
   *num_classes_host = nc;
   return HPX_OK;
 }
+
+}  // namespace
+
+extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_host,
+                                  const uint32_t* drop_index_host, int64_t num_drops, uint64_t seed,
+                                  const double* noise_power, int32_t* errors, void* plans_host, int64_t* order_host,
+                                  hpx_uneven_class* classes_host, int max_classes, int* num_classes_host) {
+  return uneven_prepare(links_host, tables_host, drop_index_host, num_drops, seed, noise_power, errors, plans_host,
+                        order_host, classes_host, max_classes, num_classes_host, nullptr, 0);
+}
+
 
 extern "C" int hpx_uneven_run(const void* plans, const hpx_uneven_class* classes_host, int num_classes,
                               void* stream) {

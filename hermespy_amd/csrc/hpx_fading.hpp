@@ -38,32 +38,36 @@ __device__ inline float2 tap_direct(const double* sos, int terms, int64_t m) {
   return acc;
 }
 
-// Taylor coefficients of one path over the block centred at mc with half width hw.
+// Taylor coefficients of one path over the block centred at mc with half width hw.  The phase
+// argument rate * mc + phase is accumulated and reduced in float64; each term's sine and cosine
+// and the sums are float32 (the reduced phase lies in [-pi, pi], so float32 sincos keeps a
+// relative error near 1e-7 per term, far below the float32 signal path's 1e-5).
 __device__ inline void taylor_coeffs(const double* sos, int terms, double mc, double hw, float2* out) {
-  double cr[kTaylorCoeffs], ci[kTaylorCoeffs];
+  float cr[kTaylorCoeffs], ci[kTaylorCoeffs];
 #pragma unroll
-  for (int k = 0; k < kTaylorCoeffs; ++k) cr[k] = ci[k] = 0.0;
+  for (int k = 0; k < kTaylorCoeffs; ++k) cr[k] = ci[k] = 0.0f;
   for (int n = 0; n < terms; ++n) {
     const double rate = sos[3 * n], amp = sos[3 * n + 2];
     if (amp == 0.0) continue;
-    const double ph = reduce_phase(fma(rate, mc, sos[3 * n + 1]));
-    double s, c;
-    sincos(ph, &s, &c);
-    double tr = amp * c, ti = amp * s;
-    const double u = rate * hw;
+    const float ph = static_cast<float>(reduce_phase(fma(rate, mc, sos[3 * n + 1])));
+    float s, c;
+    sincosf(ph, &s, &c);
+    const float a = static_cast<float>(amp);
+    float tr = a * c, ti = a * s;
+    const float u = static_cast<float>(rate * hw);
 #pragma unroll
     for (int k = 0; k < kTaylorCoeffs; ++k) {
       cr[k] += tr;
       ci[k] += ti;
       // term *= j u / (k + 1)
-      const double f = u / static_cast<double>(k + 1);
-      const double nr = -ti * f, ni = tr * f;
+      const float f = u * (1.0f / static_cast<float>(k + 1));
+      const float nr = -ti * f, ni = tr * f;
       tr = nr;
       ti = ni;
     }
   }
 #pragma unroll
-  for (int k = 0; k < kTaylorCoeffs; ++k) out[k] = make_float2(static_cast<float>(cr[k]), static_cast<float>(ci[k]));
+  for (int k = 0; k < kTaylorCoeffs; ++k) out[k] = make_float2(cr[k], ci[k]);
 }
 
 __device__ __forceinline__ float2 taylor_eval(const float2* c, float s) {
@@ -89,6 +93,11 @@ struct TaylorBlocks {
   // of magnitude < 2^12 over B), which the single rounding of the FMA therefore keeps
   __device__ __forceinline__ float offset(int64_t m) const {
     const int r = static_cast<int>(m & ((int64_t(1) << log2b) - 1));  // 0 .. B-1
+    return fmaf(static_cast<float>(r), ldexpf(2.0f, -log2b), ldexpf(1.0f, -log2b) - 1.0f);
+  }
+  // the same for a sample index that fits 32 bits
+  __device__ __forceinline__ float offset32(int m) const {
+    const int r = m & ((1 << log2b) - 1);
     return fmaf(static_cast<float>(r), ldexpf(2.0f, -log2b), ldexpf(1.0f, -log2b) - 1.0f);
   }
 };
@@ -120,20 +129,21 @@ __device__ inline double2 tap_mean(const double* sos, int terms, double m0, int 
 }
 
 // tap_mean with the per-term weights amp_dir[k] = amp * dirichlet_factor(rate, n) computed once
-// for every word (the same value, so the same result)
-__device__ inline double2 tap_mean_weighted(const double* sos, const double* amp_dir, int terms, double m0, int n) {
+// for every word; the phase is reduced in float64, its sine and cosine and the sum are float32
+// (as taylor_coeffs).
+__device__ inline float2 tap_mean_weighted(const double* sos, const double* amp_dir, int terms, double m0, int n) {
   const double centre = m0 + 0.5 * static_cast<double>(n - 1);
-  double ar = 0.0, ai = 0.0;
+  float ar = 0.0f, ai = 0.0f;
   for (int k = 0; k < terms; ++k) {
     if (sos[3 * k + 2] == 0.0) continue;
-    const double w = amp_dir[k];
-    const double ph = reduce_phase(fma(sos[3 * k], centre, sos[3 * k + 1]));
-    double s, c;
-    sincos(ph, &s, &c);
-    ar += w * c;
-    ai += w * s;
+    const float w = static_cast<float>(amp_dir[k]);
+    const float ph = static_cast<float>(reduce_phase(fma(sos[3 * k], centre, sos[3 * k + 1])));
+    float s, c;
+    sincosf(ph, &s, &c);
+    ar = fmaf(w, c, ar);
+    ai = fmaf(w, s, ai);
   }
-  return make_double2(ar, ai);
+  return make_float2(ar, ai);
 }
 
 }  // namespace hpx

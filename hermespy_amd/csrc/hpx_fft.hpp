@@ -101,6 +101,214 @@ __device__ __forceinline__ void lds_fft(float2* data, int count, int stride, int
   }
 }
 
+// ---------------------------------------------------------------- mixed-radix variant
+// Stockham passes of radix 16 (16-point DFTs in registers as 4 x 4), then one radix-4 and/or one
+// radix-2 pass for the remaining factor (N = 1024: 16 16 4; 4096: 16 16 16): fewer passes, LDS
+// round trips and barriers than radix 4 throughout.  A radix-R pass of stride ns reads
+// x[j + k N/R] (consecutive lanes, consecutive elements), multiplies by the twiddles
+// exp(-2 pi j k pos / (R ns)), pos = j mod ns, and writes output r to (j - pos) R + pos + r ns;
+// for small ns those writes put consecutive lanes 16 elements apart, so the intermediate
+// vectors live in a padded layout, element i at i + (i >> 5), which spreads them over the LDS
+// banks (the first pass reads and the last pass writes the natural layout).  Twiddles come from
+// a pass table: pass p's entries (R - 1) ns follow the previous passes', entry (k - 1) ns + pos,
+// so consecutive lanes read consecutive entries; N - 1 entries in all.  A vector's padded
+// extent is N + N / 32 elements: the stride between vectors must cover it (kFftNoPad: N).
+
+constexpr int kFftPadShift = 5;
+constexpr int kFftNoPad = 30;  // i >> 30 == 0 for every index of a vector
+
+__device__ __forceinline__ int fft_pad(int i, int shift) { return i + (i >> shift); }
+
+// Radix of pass p of an N = 2^log2n transform (16 while 4 bits remain, then 4, then 2).
+__host__ __device__ __forceinline__ int fft_pass_log2_radix(int log2n, int p) {
+  const int full = log2n >> 2;
+  if (p < full) return 4;
+  const int rem = log2n - 4 * full;
+  if (p == full) return rem >= 2 ? 2 : rem;
+  return (p == full + 1 && rem == 3) ? 1 : 0;
+}
+
+__host__ __device__ __forceinline__ int fft_num_passes(int log2n) {
+  int p = 0;
+  while (fft_pass_log2_radix(log2n, p) > 0) ++p;
+  return p;
+}
+
+// Entries of the pass table of an N-point transform (N - 1).
+__host__ __device__ __forceinline__ int fft_pass_table_size(int n) { return n - 1; }
+
+// Fills the pass table of an N-point transform (N a power of two >= 2), float32 from float64;
+// called by every thread of the workgroup (no barrier).
+__device__ __forceinline__ void fill_fft_pass_table(float2* tab, int log2n) {
+  int ns = 1, off = 0;
+  for (int p = 0;; ++p) {
+    const int lr = fft_pass_log2_radix(log2n, p);
+    if (lr == 0) break;
+    const int r = 1 << lr;
+    for (int e = threadIdx.x; e < (r - 1) * ns; e += blockDim.x) {
+      const int k = 1 + e / ns, pos = e - (k - 1) * ns;
+      double sn, cs;
+      sincospi(-2.0 * static_cast<double>(k * pos) / static_cast<double>(r * ns), &sn, &cs);
+      tab[off + e] = make_float2(static_cast<float>(cs), static_cast<float>(sn));
+    }
+    off += (r - 1) * ns;
+    ns *= r;
+  }
+}
+
+template <int SIGN>
+__device__ __forceinline__ float2 pass_twiddle(const float2* tab, int idx) {
+  const float2 w = tab[idx];
+  return SIGN < 0 ? w : make_float2(w.x, -w.y);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3);
+  const float2 t3 = mul_sign_i<SIGN>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a1 = cadd(t1, t3);
+  a2 = csub(t0, t2);
+  a3 = csub(t1, t3);
+}
+
+// x * exp(SIGN 2 pi j m / 16) for the products m = n2 k1 of the 4 x 4 split (0, 1, 2, 3, 4, 6, 9)
+template <int SIGN, int M>
+__device__ __forceinline__ float2 w16(float2 x) {
+  constexpr float c[10] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.0f,
+                           -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f, -1.0f,
+                           -0.92387953251128674f};
+  constexpr float sn[10] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f, 1.0f,
+                            0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.0f,
+                            -0.38268343236508977f};
+  if (M == 0) return x;
+  if (M == 4) return mul_sign_i<SIGN>(x);
+  return cmul(x, make_float2(c[M], SIGN * sn[M]));
+}
+
+// In-place 16-point DFT of v[0..15] (sign SIGN), output X[k] in v[k]: X[k1 + 4 k2] =
+// sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 x[4 n1 + n2] W4^(n1 k1).
+template <int SIGN>
+__device__ __forceinline__ void dft16(float2* v) {
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) dft4<SIGN>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);  // v[4 k1 + n2]
+  v[4 + 1] = w16<SIGN, 1>(v[4 + 1]);
+  v[4 + 2] = w16<SIGN, 2>(v[4 + 2]);
+  v[4 + 3] = w16<SIGN, 3>(v[4 + 3]);
+  v[8 + 1] = w16<SIGN, 2>(v[8 + 1]);
+  v[8 + 2] = w16<SIGN, 4>(v[8 + 2]);
+  v[8 + 3] = w16<SIGN, 6>(v[8 + 3]);
+  v[12 + 1] = w16<SIGN, 3>(v[12 + 1]);
+  v[12 + 2] = w16<SIGN, 6>(v[12 + 2]);
+  v[12 + 3] = w16<SIGN, 9>(v[12 + 3]);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) dft4<SIGN>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);  // -> v[4 k1 + k2]
+  // transpose 4 x 4: X[k1 + 4 k2] currently sits at v[4 k1 + k2]
+  float2 t;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = i + 1; j < 4; ++j) {
+      t = v[4 * i + j];
+      v[4 * i + j] = v[4 * j + i];
+      v[4 * j + i] = t;
+    }
+}
+
+template <int SIGN, int LOG2R, int MAXV, bool PIN, bool POUT>
+__device__ __forceinline__ void fft_pass_mixed(float2* data, int count, int stride, int log2n, int log2ns,
+                                               const float2* tab, int shift) {
+  constexpr int R = 1 << LOG2R;
+  constexpr int kItems = MAXV / R > 0 ? MAXV / R : 1;
+  const int log2nr = log2n - LOG2R;
+  const int nr = 1 << log2nr;
+  const int ns = 1 << log2ns;
+  const int total = count << log2nr;
+  // the thread index passes through an opaque copy: the per-thread index arithmetic is redone in
+  // every pass instead of being hoisted out of the caller's loops into registers held throughout
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  float2 v[kItems][R];
+  int dst[kItems];
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const int item = tid + it * blockDim.x;
+    if (item < total) {
+      const float2* base = data + (item >> log2nr) * stride;
+      const int j = item & (nr - 1);
+      const int pos = j & (ns - 1);
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int i = j + k * nr;
+        v[it][k] = base[PIN ? fft_pad(i, shift) : i];
+      }
+      if (ns > 1) {
+        const float2* ptab = tab + (ns - 1);  // passes before this one hold ns - 1 entries
+#pragma unroll
+        for (int k = 1; k < R; ++k) v[it][k] = cmul(v[it][k], pass_twiddle<SIGN>(ptab, (k - 1) * ns + pos));
+      }
+      if (R == 16) {
+        dft16<SIGN>(v[it]);
+      } else if (R == 4) {
+        dft4<SIGN>(v[it][0], v[it][1 % R], v[it][2 % R], v[it][3 % R]);
+      } else {
+        const float2 a0 = v[it][0], a1 = v[it][1 % R];
+        v[it][0] = cadd(a0, a1);
+        v[it][1 % R] = csub(a0, a1);
+      }
+      dst[it] = ((j - pos) << LOG2R) + pos;  // within the vector
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const int item = tid + it * blockDim.x;
+    if (item < total) {
+      float2* row = data + (item >> log2nr) * stride;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int i = dst[it] + r * ns;
+        row[POUT ? fft_pad(i, shift) : i] = v[it][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int SIGN, int LOG2R, int MAXV>
+__device__ __forceinline__ void fft_pass_layout(float2* data, int count, int stride, int log2n, int log2ns,
+                                                const float2* tab, int shift, bool first, bool last) {
+  if (first && last) fft_pass_mixed<SIGN, LOG2R, MAXV, false, false>(data, count, stride, log2n, log2ns, tab, shift);
+  else if (first) fft_pass_mixed<SIGN, LOG2R, MAXV, false, true>(data, count, stride, log2n, log2ns, tab, shift);
+  else if (last) fft_pass_mixed<SIGN, LOG2R, MAXV, true, false>(data, count, stride, log2n, log2ns, tab, shift);
+  else fft_pass_mixed<SIGN, LOG2R, MAXV, true, true>(data, count, stride, log2n, log2ns, tab, shift);
+}
+
+// In-place complex FFT (sign SIGN, no scaling) of `count` vectors at data + v * stride in LDS,
+// natural order in and out, with the pass table `tab` (fill_fft_pass_table) and the padded
+// intermediate layout of pad shift `shift`.  Called by every thread; ends with a barrier.
+// Vectors are transformed in groups of at most MAXV * blockDim.x / N.
+template <int SIGN, int MAXV = kFftMaxPerThread>
+__device__ __forceinline__ void lds_fft_padded(float2* data, int count, int stride, int log2n, const float2* tab,
+                                               int shift = kFftPadShift) {
+  static_assert(MAXV % 4 == 0, "MAXV must be a multiple of 4");
+  const int group = max(1, (MAXV * static_cast<int>(blockDim.x)) >> log2n);
+  const int passes = fft_num_passes(log2n);
+  for (int g0 = 0; g0 < count; g0 += group) {
+    const int cnt = min(group, count - g0);
+    float2* base = data + g0 * stride;
+    int log2ns = 0;
+    for (int p = 0; p < passes; ++p) {
+      const int lr = fft_pass_log2_radix(log2n, p);
+      const bool first = p == 0, last = p == passes - 1;
+      if (lr == 4) fft_pass_layout<SIGN, 4, MAXV>(base, cnt, stride, log2n, log2ns, tab, shift, first, last);
+      else if (lr == 2) fft_pass_layout<SIGN, 2, MAXV>(base, cnt, stride, log2n, log2ns, tab, shift, first, last);
+      else fft_pass_layout<SIGN, 1, MAXV>(base, cnt, stride, log2n, log2ns, tab, shift, first, last);
+      log2ns += lr;
+    }
+  }
+}
+
 // tw[k] = exp(-2 pi j k / N), k < N, float32 from float64, called by every thread of the
 // workgroup (no barrier).  Computes the N / 8 + 1 angles of the first octant and fills the
 // other seven by the symmetries cos(pi/2 - x) = sin(x) and exp(-j (x + q pi/2)) = (-j)^q exp(-j x),

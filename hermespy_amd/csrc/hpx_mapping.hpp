@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <string.h>
 
 namespace hpx {
 
@@ -101,6 +102,103 @@ __host__ __device__ inline uint32_t demap_symbol(const Modulation& m, double re,
   const double ri = m.pam == 2 ? im * m.pre : (im * m.pre) * m.scale;
   const uint32_t br = pam_decide(m.pam, rr, 1);
   const uint32_t bi = pam_decide(m.pam, ri, 1);
+  const int nb = m.bps / 2;
+  uint32_t out = 0;
+  for (int i = 0; i < nb; ++i) {
+    out |= ((br >> i) & 1u) << (2 * i);
+    out |= ((bi >> i) & 1u) << (2 * i + 1);
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------- float32 inputs
+// For a float32 component x of a complex constellation, r(x) = (x * pre) * scale (or x * pre for
+// QPSK) is formed in float64 and rounded monotonically, so every decision "r > T" or "r < T" of
+// pam_decide is "x > a_T" or "x < b_T" for two float32 boundaries: a_T the largest x >= 0 with
+// r(x) <= T, b_T the smallest x >= 0 with r(x) >= T (r is odd, so -T uses -a_T and -b_T).  The
+// sign decision is x < 0: for a finite float32 x the reference's t of pam_sign_bit cannot underflow
+// to zero unless x is zero.  demap_symbol_f32 with these boundaries therefore decides exactly what
+// demap_symbol decides on the float64 value of x, using float32 comparisons only.
+struct DemapBounds {
+  float a[8], b[8];  // thresholds T = 2, 4, ..., 16 (index T / 2 - 1)
+};
+
+__host__ __device__ inline double pam_amplitude_of(const Modulation& m, float x) {
+  return m.pam == 2 ? static_cast<double>(x) * m.pre : (static_cast<double>(x) * m.pre) * m.scale;
+}
+
+// Host: boundaries of a complex modulation by bisection over the float32 bit patterns of x >= 0.
+inline DemapBounds demap_bounds(const Modulation& m) {
+  DemapBounds d{};
+  auto as_float = [](uint32_t u) {
+    float f;
+    memcpy(&f, &u, sizeof(f));
+    return f;
+  };
+  const uint32_t top = 0x7f7fffffu;  // largest finite float32
+  for (int i = 0; i < 8; ++i) {
+    const double t = 2.0 * (i + 1);
+    // a_T: largest x with r(x) <= T
+    uint32_t lo = 0, hi = top;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo + 1) / 2;
+      if (pam_amplitude_of(m, as_float(mid)) <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    d.a[i] = as_float(lo);
+    // b_T: smallest x with r(x) >= T
+    lo = 0;
+    hi = top;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (pam_amplitude_of(m, as_float(mid)) >= t) hi = mid;
+      else lo = mid + 1;
+    }
+    d.b[i] = as_float(lo);
+  }
+  return d;
+}
+
+// Bits of one PAM axis of a complex constellation for a float32 component x (as pam_decide on r(x));
+// d points at a DemapBounds (in LDS on the device, read where a comparison needs a boundary).
+__host__ __device__ inline uint32_t pam_decide_f32(int pam, float x, const DemapBounds* d) {
+  if (!isfinite(x)) return 0u;
+  // r > T: x > a[T/2-1]; r < T: x < b[T/2-1]; r < -T: x < -a; r > -T: x > -b
+#define HPX_GT(T) (x > d->a[(T) / 2 - 1])
+#define HPX_LT(T) (x < d->b[(T) / 2 - 1])
+#define HPX_LTN(T) (x < -d->a[(T) / 2 - 1])
+#define HPX_GTN(T) (x > -d->b[(T) / 2 - 1])
+  uint32_t b = x < 0.0f ? 1u : 0u;
+  switch (pam) {
+    case 2:
+      break;
+    case 4:
+      b |= (HPX_LTN(2) || HPX_GT(2)) ? 2u : 0u;
+      break;
+    case 8:
+      b |= (HPX_LTN(4) || HPX_GT(4)) ? 2u : 0u;
+      b |= (HPX_LTN(6) || (HPX_GTN(2) && HPX_LT(2)) || HPX_GT(6)) ? 4u : 0u;
+      break;
+    default:  // 16
+      b |= (HPX_LTN(8) || HPX_GT(8)) ? 2u : 0u;
+      b |= (HPX_LTN(12) || (HPX_GTN(4) && HPX_LT(4)) || HPX_GT(12)) ? 4u : 0u;
+      b |= (HPX_LTN(14) || (HPX_GTN(10) && HPX_LTN(6)) || (HPX_GTN(2) && HPX_LT(2)) || (HPX_GT(6) && HPX_LT(10)) ||
+            HPX_GT(14))
+               ? 8u
+               : 0u;
+      break;
+  }
+#undef HPX_GT
+#undef HPX_LT
+#undef HPX_LTN
+#undef HPX_GTN
+  return b;
+}
+
+// demap_symbol of a complex constellation for float32 components (bit-identical decisions).
+__host__ __device__ inline uint32_t demap_symbol_f32(const Modulation& m, const DemapBounds* d, float re, float im) {
+  const uint32_t br = pam_decide_f32(m.pam, re, d);
+  const uint32_t bi = pam_decide_f32(m.pam, im, d);
   const int nb = m.bps / 2;
   uint32_t out = 0;
   for (int i = 0; i < nb; ++i) {

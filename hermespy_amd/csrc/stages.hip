@@ -330,6 +330,22 @@ extern "C" int hpx_demap_hard_host(const hpx_c128* symbols_host, int64_t num_sym
   return HPX_OK;
 }
 
+extern "C" int hpx_demap_hard_f32_host(const hpx_c64* symbols_host, int64_t num_symbols, int modulation_order,
+                                       uint8_t* bits_host) {
+  HPX_REQUIRE(symbols_host && bits_host, "hpx_demap_hard_f32_host: null pointer");
+  Modulation m;
+  if (modulation_order == 2 || !modulation(modulation_order, 1, &m)) {
+    set_error("Unsupported modulation scheme");
+    return HPX_ERR_UNSUPPORTED;
+  }
+  const DemapBounds d = demap_bounds(m);
+  for (int64_t i = 0; i < num_symbols; ++i) {
+    const uint32_t b = demap_symbol_f32(m, &d, symbols_host[i].re, symbols_host[i].im);
+    for (int k = 0; k < m.bps; ++k) bits_host[i * m.bps + k] = static_cast<uint8_t>((b >> k) & 1u);
+  }
+  return HPX_OK;
+}
+
 __global__ void __launch_bounds__(256) map_kernel(const uint8_t* __restrict__ bits, int64_t nsym, int order,
                                                   int is_complex, int bps, float2* __restrict__ out) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -190,6 +190,53 @@ class BatchedSimulation:
         self._replay_file = None
         self._drop_counter = 0
         self.replay_mismatches = 0
+        self._num_samples = 100  # Simulation default (simulation.py:260-264)
+        self._snr_db: tuple | None = None
+        self._evaluators: list = []
+
+    # ---------------------------------------------------------------- the reference's grid form
+    @property
+    def num_samples(self) -> int:
+        """Drops per grid section (``MonteCarlo.num_samples``, ``monte_carlo.py:587-605``)."""
+
+        return self._num_samples
+
+    @num_samples.setter
+    def num_samples(self, value: int) -> None:
+        if value < 1:
+            raise ValueError("Number of samples must be greater than zero")
+        self._num_samples = int(value)
+
+    def new_dimension(self, dimension: str, sample_points, *args, **kwargs):
+        """``MonteCarlo.new_dimension`` (``monte_carlo.py:503-534``) for the dimension this path
+        sweeps, the receiver's ``noise_level`` (``snr`` is accepted as well): sample points are SNRs as
+        power ratios, or logarithmic values carrying ``value_db`` (the reference's ``dB(...)``).
+        Further arguments (the devices the dimension applies to) are accepted and not needed."""
+
+        if dimension not in ("noise_level", "snr"):
+            raise ValueError(f"only the noise level (SNR) dimension runs on the batched path, not '{dimension}'")
+        pts = list(sample_points)
+        if not pts:
+            raise ValueError("a dimension needs at least one sample point")
+        snr = []
+        for v in pts:
+            if hasattr(v, "value_db"):
+                snr.append(float(v.value_db))
+            elif float(v) > 0.0:
+                snr.append(10.0 * float(np.log10(float(v))))
+            else:
+                raise ValueError("SNR sample points must be positive power ratios")
+        self._snr_db = tuple(snr)
+        from .campaign import snr_dimension
+
+        return snr_dimension(self._snr_db)
+
+    def add_evaluator(self, evaluator) -> None:
+        """``MonteCarlo.add_evaluator`` (``monte_carlo.py:572-580``): the bit error evaluator."""
+
+        if getattr(evaluator, "abbreviation", None) != "BER":
+            raise ValueError("the batched drop path evaluates bit errors (BitErrorEvaluator)")
+        self._evaluators.append(evaluator)
 
     # ---------------------------------------------------------------- record / replay
     @property
@@ -292,11 +339,19 @@ class BatchedSimulation:
 
     def run(self, num_drops: int | None = None, snr_db=None, drops_per_launch: int = 4096,
             first_drop: int = 0) -> BatchResult:
-        """num_drops drops, drop i at SNR point i % len(snr_db).  In record mode the drops are
+        """Without arguments (default mode): ``Simulation.run()``, the campaign over the grid set by
+        :meth:`new_dimension` (the link's SNR points otherwise), ``num_samples`` drops per section
+        with the stopping rule of the evaluator added by :meth:`add_evaluator`, returning a
+        ``MonteCarloResult``.  Otherwise num_drops drops, drop i at SNR point i % len(snr_db).  In record mode the drops are
         recorded; in replay mode the next num_drops recorded drops (all remaining ones if None)
         are simulated again from their recorded inputs, and ``replay_mismatches`` counts drops
         whose error count differs from the recorded one."""
 
+        if num_drops is None and snr_db is None and self._mode == ScenarioMode.DEFAULT:
+            # Simulation.run() (simulation.py:408-457): the configured grid, num_samples drops per
+            # section with the evaluator's stopping rule, as a MonteCarloResult
+            return self.simulate(self.num_samples, self._evaluators[0] if self._evaluators else None,
+                                 self._snr_db)
         snr = np.asarray(self.link.snr_db if snr_db is None else snr_db, dtype=np.float64)
         if self._mode == ScenarioMode.REPLAY:
             return self._run_replay(num_drops, snr, drops_per_launch)

@@ -139,8 +139,8 @@ class UnevenBatch:
         classes = (_lib.HpxUnevenClass * MAX_CLASSES)()
         nc = _lib.c_int(0)
         _lib.check(self.lib.hpx_uneven_prepare(links, self.tables, idx.ctypes.data, n, ctypes.c_uint64(self.seed),
-                                               self.noise_power.data_ptr(), self.errors.data_ptr(), plans.ctypes.data,
-                                               order.ctypes.data, classes, MAX_CLASSES, ctypes.byref(nc)))
+                           _lib.ptr(self.noise_power, "noise_power"), _lib.ptr(self.errors, "errors"), plans.ctypes.data,
+                           order.ctypes.data, classes, MAX_CLASSES, ctypes.byref(nc)))
         self.plans = torch.as_tensor(plans, device=self.device)
         self.order = order
         self.num_classes = nc.value

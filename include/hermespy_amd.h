@@ -131,6 +131,12 @@ int hpx_constellation_host(int modulation_order, int is_complex, hpx_c128* point
 int hpx_demap_hard_host(const hpx_c128* symbols_host, int64_t num_symbols, int modulation_order, int is_complex,
                         uint8_t* bits_host);
 
+/* Host twin of the float32 decision path the detection kernels use for complex constellations
+ * (4, 16, 64, 256): every threshold of the reference's piecewise LLRs replaced by the float32
+ * boundary of the float64 amplitude it is compared in (psk_qam_mapping.py:301-497); the same
+ * decisions as hpx_demap_hard_host on the float64 values of the inputs. */
+int hpx_demap_hard_f32_host(const hpx_c64* symbols_host, int64_t num_symbols, int modulation_order, uint8_t* bits_host);
+
 /* OFDM modulation of a [num_streams][words][num_subcarriers] grid into
  * [num_streams][ld_out] frames of words * (num_subcarriers + prefix) samples,
  * oversampling factor 1.  Replaces OFDMWaveform._forward_transformation +
@@ -464,6 +470,7 @@ int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables
 
 /* One launch per class from the device copy of the plans. */
 int hpx_uneven_run(const void* plans, const hpx_uneven_class* classes_host, int num_classes, void* stream);
+
 
 /* ---------------------------------------------------------------- massive MIMO
  * Uplink drops with many receive antennas (<= 64) and single-antenna users (<= 8): beyond the

@@ -193,3 +193,40 @@ def test_massive_run_checks_its_buffers(native_lib):
         batch.run(1, 0, torch.zeros(2, dtype=torch.float32), torch.zeros(2, dtype=torch.int32))
     with pytest.raises(ValueError, match="one entry per drop"):
         batch.run(1, 0, torch.zeros(2, dtype=torch.float64), torch.zeros(1, dtype=torch.int32))
+
+
+@pytest.mark.parametrize("order", [4, 16, 64, 256])
+def test_host_demap_f32_matches_f64(native_lib, order):
+    """The float32 decision path (boundaries found by bisection of the float64 amplitude) decides
+    exactly as the float64 demapper on the same float32 inputs: random points, every threshold's
+    float32 neighbours (8 ulps each side), signed zeros, the largest floats and non-finite values."""
+
+    rng = np.random.default_rng(order)
+    pts = mapping.constellation(order, True)
+    lim = 1.5 * np.max(np.abs(pts.real))
+    x = (rng.uniform(-lim, lim, 20000) + 1j * rng.uniform(-lim, lim, 20000)).astype(np.complex64)
+    pam = {4: 2, 16: 4, 64: 8, 256: 16}[order]
+    scale = np.sqrt(2) * {2: 1.0, 4: np.sqrt(5), 8: np.sqrt(21), 16: np.sqrt(85)}[pam]
+    thr = (np.arange(-pam, pam + 1, dtype=np.float64) / scale).astype(np.float32)
+    edge = [thr]
+    for _ in range(8):
+        edge.append(np.nextafter(edge[-1], np.float32(np.inf)))
+    down = [thr]
+    for _ in range(8):
+        down.append(np.nextafter(down[-1], np.float32(-np.inf)))
+    e = np.concatenate(edge + down + [np.array([0.0, -0.0, 3.4e38, -3.4e38, np.inf, -np.inf, np.nan, 1e-45,
+                                                -1e-45], dtype=np.float32)])
+    x = np.concatenate([x, (e + 1j * e[::-1]).astype(np.complex64), (e + 0.3j).astype(np.complex64),
+                        (0.3 + 1j * e).astype(np.complex64)])
+    bps = int(np.log2(order))
+    got = np.zeros(x.size * bps, dtype=np.uint8)
+    xx = np.ascontiguousarray(np.stack([x.real, x.imag], -1).astype(np.float32))
+    assert native_lib.hpx_demap_hard_f32_host(xx.ctypes.data, x.size, order, got.ctypes.data) == 0
+    exp = np.zeros_like(got)
+    x64 = np.ascontiguousarray(np.stack([x.real, x.imag], -1).astype(np.float64))
+    assert native_lib.hpx_demap_hard_host(x64.ctypes.data, x.size, order, 1, exp.ctypes.data) == 0
+    assert np.array_equal(got, exp)
+    # and the float64 demapper is the oracle's restatement of the reference (finite inputs)
+    fin = np.isfinite(x.real) & np.isfinite(x.imag)
+    ref = mapping.detect_bits(x[fin].astype(np.complex128), order, True).astype(np.uint8)
+    assert np.array_equal(got.reshape(-1, bps)[fin].reshape(-1), ref)

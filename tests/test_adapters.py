@@ -118,3 +118,34 @@ def test_equalizers_reshape_stated_symbols(monkeypatch, nrx, ntx):
     np.testing.assert_allclose(out.raw.numpy(), exp, rtol=1e-4, atol=1e-5)
     mm = MMSEChannelEqualization(0.0).equalize_channel(stated)
     assert mm.raw.shape == exp.shape
+
+
+def test_simulation_grid_form_arguments():
+    """``new_dimension`` / ``num_samples`` / ``add_evaluator`` of the reference's ``MonteCarlo``
+    (``monte_carlo.py:503-605``): SNR points as power ratios or dB-carrying values, and the
+    reference's argument errors."""
+
+    from hermespy_amd.configs import siso_64
+    from hermespy_amd.modem import BitErrorEvaluator
+    from hermespy_amd.simulation import BatchedSimulation
+
+    sim = BatchedSimulation(siso_64(), device="cpu")
+    dim = sim.new_dimension("noise_level", [1.0, 10.0, 100.0])
+    assert np.allclose(sim._snr_db, [0.0, 10.0, 20.0]) and dim.num_sample_points == 3
+
+    class _Db(float):
+        value_db = 7.0
+
+    sim.new_dimension("noise_level", [_Db(5.0)])
+    assert sim._snr_db == (7.0,)
+    with pytest.raises(ValueError, match="noise level"):
+        sim.new_dimension("carrier_frequency", [1e9])
+    with pytest.raises(ValueError, match="positive"):
+        sim.new_dimension("noise_level", [0.0])
+    with pytest.raises(ValueError, match="greater than zero"):
+        sim.num_samples = 0
+    sim.num_samples = 64
+    assert sim.num_samples == 64
+    with pytest.raises(ValueError, match="bit errors"):
+        sim.add_evaluator(object())
+    sim.add_evaluator(BitErrorEvaluator())

@@ -117,3 +117,18 @@ def test_campaign_stops_like_the_sequential_oracle(cuda):
     pooled = r.pooled_ber()
     for s in range(S):
         np.testing.assert_allclose(pooled[s], e[np.arange(counts[s]) * S + s].sum() / (counts[s] * bits), rtol=1e-15)
+    # the reference's grid form runs the same campaign: new_dimension + num_samples + add_evaluator + run()
+    class _Db(float):
+        def __new__(cls, db):
+            obj = float.__new__(cls, 10 ** (db / 10))
+            obj.value_db = float(db)
+            return obj
+
+    grid = BatchedSimulation(cfg, seed=9)
+    grid.new_dimension("noise_level", [_Db(v) for v in snr])
+    grid.num_samples = K
+    grid.add_evaluator(ev)
+    res2 = grid.run()
+    r2 = res2.evaluation_results[0]
+    assert np.array_equal(r2.num_drops(), r.num_drops())
+    np.testing.assert_allclose(r2.to_array(), r.to_array(), rtol=1e-12, atol=0)

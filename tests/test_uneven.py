@@ -75,7 +75,7 @@ def test_planner_groups_by_kernel_class(native_lib):
 def test_lean_drops_need_a_workspace(native_lib):
     """An 8x8 link at 2048 subcarriers exceeds one CU's LDS: the single-link path refuses it, the
     heterogeneous planner sizes its workspace (Taylor table of every path and block, then N
-    twiddles) and refuses the drop without one."""
+    twiddles and the N - 1 entries of the FFT pass table) and refuses the drop without one."""
 
     from hermespy_amd import _lib
     from hermespy_amd.uneven import MAX_CLASSES, DropShape, _c_link
@@ -83,9 +83,10 @@ def test_lean_drops_need_a_workspace(native_lib):
     big = _c_link(DropShape(8, 2048, 23, 2.0, 100.0, 16, 10.0).link())
     assert native_lib.hpx_run_batch_lds_bytes(ctypes.byref(big)) == 0
     ws = native_lib.hpx_uneven_workspace_elements(ctypes.byref(big))
-    assert ws > 2048 and (ws - 2048) % (23 * 5) == 0
+    tables_len = 2048 + 2047
+    assert ws > tables_len and (ws - tables_len) % (23 * 5) == 0
     fewer = _c_link(DropShape(8, 2048, 5, 2.0, 100.0, 16, 10.0).link())
-    assert (native_lib.hpx_uneven_workspace_elements(ctypes.byref(fewer)) - 2048) * 23 == (ws - 2048) * 5
+    assert (native_lib.hpx_uneven_workspace_elements(ctypes.byref(fewer)) - tables_len) * 23 == (ws - tables_len) * 5
     for shape in ((8, 1024), (4, 2048), (1, 128)):
         small = _c_link(DropShape(shape[0], shape[1], 23, 2.0, 100.0, 16, 10.0).link())
         assert native_lib.hpx_uneven_workspace_elements(ctypes.byref(small)) == 0
