@@ -142,6 +142,10 @@ SIGNATURES = {
     "hpx_uneven_prepare": (c_int, [ctypes.POINTER(HpxLink), ctypes.POINTER(HpxDropTables), c_vp, c_i64, c_u64, c_vp,
                                    c_vp, c_vp, c_vp, ctypes.POINTER(HpxUnevenClass), c_int, ctypes.POINTER(c_int)]),
     "hpx_uneven_run": (c_int, [c_vp, ctypes.POINTER(HpxUnevenClass), c_int, c_vp]),
+    "hpx_uneven_prepare_noalloc": (c_int, [ctypes.POINTER(HpxLink), ctypes.POINTER(HpxDropTables), c_vp, c_i64, c_u64,
+                                           c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(HpxUnevenClass), c_int,
+                                           ctypes.POINTER(c_int)]),
+    "hpx_uneven_noalloc_capacity": (c_i64, []),
     "hpx_run_batch_lds_bytes": (c_i64, [ctypes.POINTER(HpxLink)]),
     "hpx_spatial_detector_host": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
     "hpx_stream_bits": (c_int, [c_u64, c_u32, c_u32, c_i64, c_vp, c_vp]),

@@ -888,7 +888,7 @@ int uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_hos
       float2* ws = reinterpret_cast<float2*>(tb.workspace);
       if (scratch_base) {
         if (scratch_used + plans[i].ws_elems > scratch_elems) {
-          set_error("hpx_uneven_prepare: the lean drops need more than the %lld complex64 values of the "
+          set_error("hpx_uneven_prepare_noalloc: the lean drops need more than the %lld complex64 values of the "
                     "library's scratch array",
                     static_cast<long long>(scratch_elems));
           return HPX_ERR_INVALID;
@@ -962,6 +962,41 @@ int uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_hos
 
 }  // namespace
 
+// Scratch of hpx_uneven_prepare_noalloc: lean drops' Taylor and twiddle tables.  Each device holds
+// its own copy until the process ends; host code never uses the name itself, only the address
+// hipGetSymbolAddress gives for the current device.
+constexpr int64_t kNoAllocScratchElems = int64_t(1) << 22;  // complex64 values: 32 MiB
+static __device__ __align__(256) float scratch[2 * kNoAllocScratchElems];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+int scratch_address(float2** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxScratchDevices) {
+    set_error("hpx_uneven_prepare_noalloc: no current HIP device");
+    return HPX_ERR_DEVICE;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hpx_uneven_prepare_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+      return HPX_ERR_DEVICE;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<float2*>(g_scratch_ptr[dev]);
+  return HPX_OK;
+}
+
+}  // namespace
+
 extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_host,
                                   const uint32_t* drop_index_host, int64_t num_drops, uint64_t seed,
                                   const double* noise_power, int32_t* errors, void* plans_host, int64_t* order_host,
@@ -970,6 +1005,34 @@ extern "C" int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tab
                         order_host, classes_host, max_classes, num_classes_host, nullptr, 0);
 }
 
+extern "C" int64_t hpx_uneven_noalloc_capacity(void) { return kNoAllocScratchElems; }
+
+extern "C" int hpx_uneven_prepare_noalloc(const hpx_link* links_host, const hpx_drop_tables* tables_host,
+                                          const uint32_t* drop_index_host, int64_t num_drops, uint64_t seed,
+                                          const double* noise_power, int32_t* errors, void* plans_host,
+                                          int64_t* order_host, hpx_uneven_class* classes_host, int max_classes,
+                                          int* num_classes_host) {
+  int64_t need = 0;
+  if (links_host && num_drops > 0) {
+    for (int64_t i = 0; i < num_drops; ++i) {
+      Plan p;
+      if (make_plan(&links_host[i], nullptr, &p, true) == HPX_OK && p.lean) need += p.ws_elems;
+    }
+  }
+  if (need > kNoAllocScratchElems) {
+    set_error("hpx_uneven_prepare_noalloc: the lean drops need %lld complex64 values of workspace, the library's "
+              "scratch array holds %lld",
+              static_cast<long long>(need), static_cast<long long>(kNoAllocScratchElems));
+    return HPX_ERR_INVALID;
+  }
+  float2* base = nullptr;
+  if (need > 0) {
+    const int st = scratch_address(&base);
+    if (st != HPX_OK) return st;
+  }
+  return uneven_prepare(links_host, tables_host, drop_index_host, num_drops, seed, noise_power, errors, plans_host,
+                        order_host, classes_host, max_classes, num_classes_host, base, kNoAllocScratchElems);
+}
 
 extern "C" int hpx_uneven_run(const void* plans, const hpx_uneven_class* classes_host, int num_classes,
                               void* stream) {

@@ -84,7 +84,10 @@ class UnevenBatch:
     first_drop + k)."""
 
     def __init__(self, shapes: list, seed: int = 20261202, first_drop: int = 0, device="cuda",
-                 workgroup_sizes: dict | None = None, streams: int = 4) -> None:
+                 workgroup_sizes: dict | None = None, streams: int = 4, no_alloc: bool = False) -> None:
+        """no_alloc: plan with ``hpx_uneven_prepare_noalloc`` (the lean drops' tables in the
+        library's own scratch array, no workspace allocated here)."""
+
         self.shapes = shapes
         self.seed = int(seed)
         self.first_drop = int(first_drop)
@@ -108,10 +111,13 @@ class UnevenBatch:
         # tables in device workspaces
         ws = np.array([_lib.check_count(self.lib.hpx_uneven_workspace_elements(ctypes.byref(_c_link(c))))
                        for c in self.links], dtype=np.int64)
-        self.workspace = torch.empty(max(1, int(ws.sum())), dtype=torch.complex64, device=dev)
-        ws_offs = np.concatenate([[0], np.cumsum(ws)[:-1]])
-        for i in np.flatnonzero(ws):
-            self.tables[i].workspace = self.workspace.data_ptr() + 8 * int(ws_offs[i])
+        self.no_alloc = bool(no_alloc)
+        self.workspace = None
+        if not self.no_alloc:
+            self.workspace = torch.empty(max(1, int(ws.sum())), dtype=torch.complex64, device=dev)
+            ws_offs = np.concatenate([[0], np.cumsum(ws)[:-1]])
+            for i in np.flatnonzero(ws):
+                self.tables[i].workspace = self.workspace.data_ptr() + 8 * int(ws_offs[i])
         self.num_lean = int(np.count_nonzero(ws))
         self.noise_power = torch.as_tensor([c.noise_power(s.snr_db) for c, s in zip(self.links, shapes)],
                                            dtype=torch.float64, device=dev)
@@ -138,7 +144,8 @@ class UnevenBatch:
         order = np.empty(n, dtype=np.int64)
         classes = (_lib.HpxUnevenClass * MAX_CLASSES)()
         nc = _lib.c_int(0)
-        _lib.check(self.lib.hpx_uneven_prepare(links, self.tables, idx.ctypes.data, n, ctypes.c_uint64(self.seed),
+        prepare = self.lib.hpx_uneven_prepare_noalloc if self.no_alloc else self.lib.hpx_uneven_prepare
+        _lib.check(prepare(links, self.tables, idx.ctypes.data, n, ctypes.c_uint64(self.seed),
                            _lib.ptr(self.noise_power, "noise_power"), _lib.ptr(self.errors, "errors"), plans.ctypes.data,
                            order.ctypes.data, classes, MAX_CLASSES, ctypes.byref(nc)))
         self.plans = torch.as_tensor(plans, device=self.device)

@@ -471,6 +471,22 @@ int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables
 /* One launch per class from the device copy of the plans. */
 int hpx_uneven_run(const void* plans, const hpx_uneven_class* classes_host, int num_classes, void* stream);
 
+/* hpx_uneven_prepare without caller workspace: the lean drops' Taylor and twiddle tables live in
+ * the library's own scratch array (a file-scope device array of hpx_uneven_noalloc_capacity()
+ * complex64 values, 32 MiB, held on every device the library runs on until the process ends);
+ * tables_host[i].workspace is ignored.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops).  Allocates no device
+ * memory; rejects (HPX_ERR_INVALID) a batch whose lean drops need more than the capacity.  All
+ * batches planned this way on one device share that array: launches of two such batches that
+ * overlap in time on one device race (the later plan overwrites the earlier one's tables).  The
+ * first call on a device looks up the array's address there (that device must be current). */
+int hpx_uneven_prepare_noalloc(const hpx_link* links_host, const hpx_drop_tables* tables_host,
+                               const uint32_t* drop_index_host, int64_t num_drops, uint64_t seed,
+                               const double* noise_power, int32_t* errors, void* plans_host, int64_t* order_host,
+                               hpx_uneven_class* classes_host, int max_classes, int* num_classes_host);
+
+/* Capacity of the scratch array of hpx_uneven_prepare_noalloc in complex64 values. */
+int64_t hpx_uneven_noalloc_capacity(void);
 
 /* ---------------------------------------------------------------- massive MIMO
  * Uplink drops with many receive antennas (<= 64) and single-antenna users (<= 8): beyond the

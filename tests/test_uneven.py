@@ -217,3 +217,32 @@ def test_lean_drops_match_the_oracle(cuda):
         ref = odrop.run_drop(oracle_link(cfg), 77, 500 + i, shape.snr_db, faithful=False)
         assert ref.errors > 0
         assert abs(int(eager[i]) - ref.errors) <= max(2, int(2e-4 * ref.bits)), (i, shape, int(eager[i]), ref.errors)
+
+
+def test_noalloc_planner_rejects_batches_beyond_its_scratch(native_lib):
+    """Lean drops needing more than the scratch array's capacity are refused before any device work."""
+
+    import ctypes
+
+    from hermespy_amd import _lib
+    from hermespy_amd.uneven import MAX_CLASSES, DropShape, _c_link
+
+    cap = int(native_lib.hpx_uneven_noalloc_capacity())
+    cfg = DropShape(8, 2048, 23, 1.5, 400.0, 16, 12.0).link()
+    per = int(native_lib.hpx_uneven_workspace_elements(ctypes.byref(_c_link(cfg))))
+    assert per > 0
+    n = cap // per + 1
+    c_links = (_lib.HpxLink * n)(*[_c_link(cfg)] * n)
+    tables = (_lib.HpxDropTables * n)()
+    for i in range(n):
+        tables[i].delays, tables[i].path_gains = 0x10, 0x20
+    idx = np.arange(n, dtype=np.uint32)
+    plans = np.empty(n * int(native_lib.hpx_uneven_plan_bytes()), dtype=np.uint8)
+    order = np.empty(n, dtype=np.int64)
+    classes = (_lib.HpxUnevenClass * MAX_CLASSES)()
+    nc = _lib.c_int(0)
+    st = native_lib.hpx_uneven_prepare_noalloc(c_links, tables, idx.ctypes.data, n, ctypes.c_uint64(0), 0x30, 0x40,
+                                               plans.ctypes.data, order.ctypes.data, classes, MAX_CLASSES,
+                                               ctypes.byref(nc))
+    with pytest.raises(ValueError, match="scratch array holds"):
+        _lib.check(st)
