@@ -6,7 +6,9 @@ reference tabulates them in ``hermespy/channel/fading/tdl.py:101-369`` and
 ``hermespy/channel/fading/cost259.py:94-202``.  The reference files are parsed
 with ``ast`` (literal arrays and scalars only; nothing is imported or run).
 
-    python hermespy_amd/data/make_fading_tables.py /root/reference
+    python tests/golden/make_fading_tables.py /root/reference
+
+The output is the product's table file, hermespy_amd/data/fading_tables.json.
 """
 
 from __future__ import annotations
@@ -78,6 +80,6 @@ def extract(root: Path) -> dict:
 if __name__ == "__main__":
     root = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
     t = extract(root)
-    out = Path(__file__).with_name("fading_tables.json")
+    out = Path(__file__).resolve().parents[2] / "hermespy_amd" / "data" / "fading_tables.json"
     out.write_text(json.dumps(t, indent=1))
     print({k: {m: len(v.get("power_db", [])) for m, v in d.items()} for k, d in t.items()})
