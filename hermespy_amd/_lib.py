@@ -16,7 +16,7 @@ import torch  # noqa: F401  (must precede the library load, see module docstring
 
 LIB_PATH = Path(__file__).resolve().parent / "_native" / "libhermespy_amd.so"
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 HPX_OK = 0
 HPX_ERR_INVALID = -1
@@ -86,7 +86,8 @@ class HpxSectionStats(ctypes.Structure):
 class HpxDropTables(ctypes.Structure):
     """Mirror of ``hpx_drop_tables`` (device pointers)."""
 
-    _fields_ = [("delays", c_vp), ("path_gains", c_vp), ("corr_rx", c_vp), ("corr_tx", c_vp), ("workspace", c_vp)]
+    _fields_ = [("delays", c_vp), ("path_gains", c_vp), ("corr_rx", c_vp), ("corr_tx", c_vp), ("workspace", c_vp),
+                ("equalized", c_vp)]
 
 
 class HpxUnevenClass(ctypes.Structure):

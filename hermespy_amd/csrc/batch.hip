@@ -904,6 +904,7 @@ int uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_hos
     }
     a.delays = tb.delays;
     a.path_gains = tb.path_gains;
+    a.equalized = reinterpret_cast<float2*>(tb.equalized);
     a.corr_rx = reinterpret_cast<const double*>(tb.corr_rx);
     a.corr_tx = reinterpret_cast<const double*>(tb.corr_tx);
     a.noise_power = noise_power + i;

@@ -84,9 +84,11 @@ class UnevenBatch:
     first_drop + k)."""
 
     def __init__(self, shapes: list, seed: int = 20261202, first_drop: int = 0, device="cuda",
-                 workgroup_sizes: dict | None = None, streams: int = 4, no_alloc: bool = False) -> None:
+                 workgroup_sizes: dict | None = None, streams: int = 4, no_alloc: bool = False,
+                 export_equalized: bool = False) -> None:
         """no_alloc: plan with ``hpx_uneven_prepare_noalloc`` (the lean drops' tables in the
-        library's own scratch array, no workspace allocated here)."""
+        library's own scratch array, no workspace allocated here).  export_equalized: keep every
+        drop's detected data symbols, ``equalized[i]`` (num_tx, words * N) complex64."""
 
         self.shapes = shapes
         self.seed = int(seed)
@@ -119,6 +121,12 @@ class UnevenBatch:
             for i in np.flatnonzero(ws):
                 self.tables[i].workspace = self.workspace.data_ptr() + 8 * int(ws_offs[i])
         self.num_lean = int(np.count_nonzero(ws))
+        self.equalized = None
+        if export_equalized:
+            self.equalized = [torch.empty((c.num_tx, c.num_data), dtype=torch.complex64, device=dev)
+                              for c in self.links]
+            for i, t in enumerate(self.equalized):
+                self.tables[i].equalized = t.data_ptr()
         self.noise_power = torch.as_tensor([c.noise_power(s.snr_db) for c, s in zip(self.links, shapes)],
                                            dtype=torch.float64, device=dev)
         self.errors = torch.zeros(n, dtype=torch.int32, device=dev)

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define HPX_ABI_VERSION 5
+#define HPX_ABI_VERSION 6
 
 typedef enum hpx_status {
   HPX_OK = 0,
@@ -440,6 +440,7 @@ typedef struct hpx_drop_tables {
   const hpx_c128* corr_tx;
   hpx_c64* workspace;        /* hpx_uneven_workspace_elements(link) elements of device memory for a
                                 drop whose link exceeds the LDS of one CU, else NULL */
+  hpx_c64* equalized;        /* [num_tx][words * N] the drop's detected data symbols, or NULL */
 } hpx_drop_tables;
 
 typedef struct hpx_uneven_class {

@@ -43,5 +43,7 @@ def oracle_links(links, seed: int, drops, snrs, faithful: bool = False, workers:
 
     tasks = [(lk, seed, int(d), float(s), faithful) for lk, d, s in zip(links, drops, snrs)]
     n = workers or max(1, min(16, os.cpu_count() or 1, len(tasks)))
+    if n == 1:
+        return [_drop(t) for t in tasks]
     with ProcessPoolExecutor(n, mp_context=mp.get_context("spawn")) as pool:
         return list(pool.map(_drop, tasks))

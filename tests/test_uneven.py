@@ -246,3 +246,38 @@ def test_noalloc_planner_rejects_batches_beyond_its_scratch(native_lib):
                                                ctypes.byref(nc))
     with pytest.raises(ValueError, match="scratch array holds"):
         _lib.check(st)
+
+
+@pytest.mark.gpu
+def test_every_kernel_class_matches_the_oracle(cuda):
+    """One drop of every (antennas, subcarriers) kernel class of the heterogeneous batch --
+    1x1 .. 8x8 at 128 .. 2048 subcarriers, the 8x8 / 2048 drop in a lean class -- with QPSK,
+    16-QAM and 64-QAM, against the float64 oracle: equalised symbols within 1e-4, exact demapping
+    of the kernel's own symbols, error counts within the decision-boundary slack."""
+
+    import torch
+
+    from conftest import oracle_link
+    from oracle_pool import oracle_links
+    from parity import check_drop
+
+    from hermespy_amd.uneven import DropShape, UnevenBatch
+
+    shapes = []
+    k = 0
+    for a in (1, 2, 4, 8):
+        for n in (128, 256, 512, 1024, 2048):
+            shapes.append(DropShape(a, n, 5 + (7 * k) % 19, 1.0 + (k % 6), 10.0 + 45.0 * k, (4, 16, 64)[k % 3],
+                                    float(3 * (k % 11))))
+            k += 1
+    print(f"{len(shapes)} kernel classes vs the oracle", flush=True)
+    ub = UnevenBatch(shapes, seed=31, first_drop=4000, export_equalized=True)
+    assert ub.num_lean == 1
+    ub.run()
+    torch.cuda.synchronize()
+    err = ub.errors.cpu().numpy()
+    eqs = [e.cpu().numpy() for e in ub.equalized]
+    links = [oracle_link(s.link()) for s in shapes]
+    ora = oracle_links(links, 31, [4000 + i for i in range(len(shapes))], [s.snr_db for s in shapes])
+    for i, s in enumerate(shapes):
+        check_drop(eqs[i], err[i], ora[i], s.modulation_order, 1e-4, (s.antennas, s.num_subcarriers))
