@@ -283,7 +283,7 @@ def campaign_leg(num_samples: int = 2048) -> dict:
     return out
 
 
-def massive_leg(drops: int = 256, chunk: int = 128) -> dict:
+def massive_leg(drops: int = 2048, chunk: int = 1024) -> dict:
     """64x8 massive-MIMO uplink drops (4096-point OFDM, 3276 active subcarriers, 256-QAM, LMMSE)."""
 
     import torch
@@ -291,7 +291,7 @@ def massive_leg(drops: int = 256, chunk: int = 128) -> dict:
     from hermespy_amd.massive import MassiveBatch, MassiveConfig
 
     cfg = MassiveConfig()
-    mb = MassiveBatch(cfg, chunk=chunk)
+    mb = MassiveBatch(cfg, chunk=chunk, keep_grid=False)
     snr = np.asarray(cfg.snr_db)
     err = torch.zeros(drops, dtype=torch.int32, device="cuda")
     npw = torch.as_tensor(cfg.noise_power(snr[np.arange(drops) % snr.size]), dtype=torch.float64, device="cuda")

@@ -16,7 +16,7 @@ import torch  # noqa: F401  (must precede the library load, see module docstring
 
 LIB_PATH = Path(__file__).resolve().parent / "_native" / "libhermespy_amd.so"
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 HPX_OK = 0
 HPX_ERR_INVALID = -1

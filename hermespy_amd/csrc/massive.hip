@@ -1,23 +1,29 @@
 // Massive-MIMO uplink drops: many receive antennas (up to 64), single-antenna users
 // (up to 8), large OFDM grids with a centred active band.  One drop does not fit a
-// CU, so the drop runs as four kernels over caller-owned per-drop buffers:
+// CU, so the drop runs as three kernels over caller-owned per-drop buffers:
 //
 //   K0 prologue  (workgroup per drop)       realisation (Philox -> Phi -> S = R_rx exp(2 pi j U) R_tx,
 //                                           SoS table), Taylor coefficients of the taps, per-word tap
-//                                           means, and the per-drop detector V, A = V^H S^H, lambda of
-//                                           S^H S (fading.py:468-515, channel_estimation.py:215-295)
+//                                           means, and the per-drop detector V, lambda of S^H S,
+//                                           A = V^H S^H and B = A S (fading.py:468-515,
+//                                           channel_estimation.py:215-295)
 //   K1 transmit  (workgroup per drop, user, word) bits -> 256-QAM -> OFDM + CP -> the user's tapped
 //                                           delay line over the word's data window (modem.py:797-888,
 //                                           ofdm.py:97-130, fading.py:371-406)
-//   K2 receive   (workgroup per drop, word, antenna group)
-//                                           y_r = sum_t S[r][t] z_t + AWGN, FFT, active bins; group 0
-//                                           also the ideal-CSI factor c of the word (ofdm.py:132-161)
-//   K3 detect    (workgroup per drop, word, 64 elements)
-//                                           LMMSE through the per-drop eigen form: u = A y on the MFMA
-//                                           units (bfloat16 hi/lo split, float32 accumulation), then
-//                                           x = V diag(conj(c) / (|c|^2 lambda + s2)) u, exact 3GPP hard
-//                                           decisions and bit errors (waveform.py:235-250,
-//                                           psk_qam_mapping.py:171-259, evaluators.py:239-258)
+//   K3 receive + detect (workgroup per drop, word)
+//                                           the matched-filter outputs u = A y of every active
+//                                           subcarrier straight from the users' delay-line outputs
+//                                           and the antennas' noise (below), the ideal-CSI factor c,
+//                                           then x = V diag(conj(c) / (|c|^2 lambda + s2)) u, exact
+//                                           3GPP hard decisions and bit errors (ofdm.py:132-161,
+//                                           waveform.py:235-250, psk_qam_mapping.py:171-259,
+//                                           evaluators.py:239-258)
+//
+// K3 never forms the per-antenna grid y_r = FFT(sum_t S[r][t] z_t + sigma n_r): the FFT is linear,
+// so u_i = sum_r A[i][r] y_r = FFT(sum_t B[i][t] z_t + sigma sum_r A[i][r] n_r) with the very noise
+// samples n_r each antenna adds in the time domain.  That is one transform per user instead of
+// one per antenna, and no grid round trip through HBM.  K2 (massive_rx_kernel) writes the
+// per-antenna grid and the CSI factor only when the caller asks for them.
 //
 // The reference caps arrays at 10 antennas (its spatial response slices one 10 x 10 block of
 // uniforms, fading.py:476-489); here the block is (max(10, Nrx), max(10, Ntx)), which keeps the
@@ -25,6 +31,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "hpx_common.hpp"
 #include "hpx_eig.hpp"
@@ -45,12 +52,11 @@ constexpr int kMaxRx = 64, kMaxUsers = 8;
 constexpr int kTxThreads = 1024, kRxThreads = 1024;
 constexpr int kRxGroup = 8;      // antennas per receive workgroup
 constexpr int kRxRows = 2;       // antennas transformed together
-constexpr int kDetElements = 64;   // resource elements per detection workgroup (32 per wave)
-constexpr int kDetThreads = 2 * kDetElements;
 constexpr int kXcds = 8;         // MI355X dispatches consecutive workgroups round robin over 8 XCDs
-
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float floatx16 __attribute__((ext_vector_type(16)));
+constexpr int kLinkThreads = 1024;
+constexpr int kLinkPairs = 2;    // sample pairs per thread of K3 (N <= 4 kLinkThreads)
+constexpr int kLinkBins = 4;     // active subcarriers per thread of K3 (active <= 4 kLinkThreads)
+constexpr int kLinkRows = 3;     // rows per transform round of K3 (users, then the CSI row)
 
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
@@ -73,16 +79,19 @@ struct MassiveArgs {
   float2* coef;     // [drop][L][nb][5]
   float2* abar;     // [drop][L][words]
   float2* det_v;    // [drop][nt][nt]
-  float2* det_a;    // [drop][nt][nr]
+  float2* det_a;    // [drop][nt][nr] A, then [nt][nt] B = A S
+  int da_stride;    // nt nr + nt nt
   float* det_l;     // [drop][nt]
   float2* z;        // [drop][nt][words][n]
-  float2* grid;     // [drop][words][nr][active]
+  float2* grid;     // [drop][words][nr][active] or null (then csi is null too)
   float2* csi;      // [drop][words][active]
   int32_t* errors;  // [drop], accumulated
   float2* equalized;  // [drop][nt][words][active] or null
   int k0_uni, k0_sos, k0_e, k0_f, k0_s64, k0_det, k0_lds;
   int k1_x, k1_tw, k1_coef, k1_bits, k1_cons, k1_dly, k1_lds;
   int k2_y, k2_tw, k2_lds;
+  int k3_rows, k3_rstride, k3_tab, k3_b, k3_a, k3_v, k3_l, k3_dmb, k3_red, k3_lds;
+  DemapBounds dbounds;  // float32 decision boundaries of mod (exact hard decisions)
 };
 
 __device__ __forceinline__ uint32_t symbol_bits_at(const uint32_t* bw, int bit, int bps) {
@@ -221,6 +230,8 @@ __global__ void __launch_bounds__(256) massive_prologue_kernel(MassiveArgs a) {
   __syncthreads();
   for (int i = tid; i < nt * nt; i += nth)
     a.det_v[ld * nt * nt + i] = make_float2(static_cast<float>(vre[i]), static_cast<float>(vim[i]));
+  double* a64 = det + nt + 2 * nt * nt;  // A [nt][nr], float64
+  float2* da = a.det_a + ld * a.da_stride;
   for (int e = tid; e < nt * nr; e += nth) {
     const int i = e / nr, r = e - i * nr;
     double ar = 0.0, ai = 0.0;
@@ -230,7 +241,21 @@ __global__ void __launch_bounds__(256) massive_prologue_kernel(MassiveArgs a) {
       ar += vr * sr - vi * si;
       ai += vr * si + vi * sr;
     }
-    a.det_a[ld * nt * nr + e] = make_float2(static_cast<float>(ar), static_cast<float>(ai));
+    a64[2 * e] = ar;
+    a64[2 * e + 1] = ai;
+    da[e] = make_float2(static_cast<float>(ar), static_cast<float>(ai));
+  }
+  __syncthreads();
+  for (int e = tid; e < nt * nt; e += nth) {
+    const int i = e / nt, t = e - i * nt;
+    double br = 0.0, bi = 0.0;
+    for (int r = 0; r < nr; ++r) {  // B[i][t] = sum_r A[i][r] S[r][t]
+      const double xr = a64[2 * (i * nr + r)], xi = a64[2 * (i * nr + r) + 1];
+      const double sr = s64[2 * (r * nt + t)], si = s64[2 * (r * nt + t) + 1];
+      br += xr * sr - xi * si;
+      bi += xr * si + xi * sr;
+    }
+    da[nt * nr + e] = make_float2(static_cast<float>(br), static_cast<float>(bi));
   }
 }
 
@@ -418,147 +443,210 @@ __global__ void __launch_bounds__(kRxThreads) massive_rx_kernel(MassiveArgs a) {
   }
 }
 
-// ---------------------------------------------------------------- K3: detection (MFMA)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<uint16_t>(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
-
-// u[e][i] = sum_r A[i][r] y[e][r] as a real GEMM: rows = elements, K = 2 nr (re parts, then im
-// parts of y), columns 0..nt-1 = Re u, nt..2nt-1 = Im u (zero-padded to 32).  Both operands are
-// split into bfloat16 hi + lo; hi*hi + hi*lo + lo*hi on 32x32x8 MFMAs keeps ~16 mantissa bits.
-constexpr int kDetK = 2 * kMaxRx;        // 128
-constexpr int kDetKs = kDetK + 4;        // padded LDS row
-
-__global__ void __launch_bounds__(kDetThreads) massive_detect_kernel(MassiveArgs a) {
-  __shared__ __align__(16) uint16_t bh[32 * kDetKs];
-  __shared__ __align__(16) uint16_t bl[32 * kDetKs];
-  __shared__ float us[kDetElements][2 * kMaxUsers + 1];
-  __shared__ int red[kDetThreads / 64];
-  const int tid = threadIdx.x;
-  const int tiles = (a.active + kDetElements - 1) / kDetElements;
+// ---------------------------------------------------------------- K3: receive + detection
+// One workgroup per (drop, word).  Each thread forms the user rows
+//   row_i = sum_t B[i][t] z_t + sigma sum_r A[i][r] n_r          (B = A S, scaled by 1 / sqrt(N))
+// at its kLinkPairs sample pairs in registers, drawing every antenna's noise (Philox stream
+// HPX_STREAM_NOISE + r, the samples massive_rx_kernel adds) and folding it in at once.  The rows
+// and the ideal-CSI row (tap means at the path delays) are then transformed kLinkRows at a time in
+// LDS; each thread keeps the active subcarriers tid + b * blockDim (b < kLinkBins) of every row in
+// registers and detects them after the last round.
+__global__ void __launch_bounds__(kLinkThreads) massive_link_kernel(MassiveArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  float2* rows = reinterpret_cast<float2*>(lds + a.k3_rows);  // [kLinkRows][rstride]
+  float2* ptab = reinterpret_cast<float2*>(lds + a.k3_tab);   // FFT pass table
+  float2* bm = reinterpret_cast<float2*>(lds + a.k3_b);       // [nt][nt] B / sqrt(N)
+  float2* am = reinterpret_cast<float2*>(lds + a.k3_a);       // [nr][kMaxUsers] sigma A^T / sqrt(N)
+  float2* vm = reinterpret_cast<float2*>(lds + a.k3_v);       // [nt][nt] V
+  float* lam = reinterpret_cast<float*>(lds + a.k3_l);
+  DemapBounds* dmb = reinterpret_cast<DemapBounds*>(lds + a.k3_dmb);
+  int* red = reinterpret_cast<int*>(lds + a.k3_red);
+  uint32_t* bits = reinterpret_cast<uint32_t*>(lds + a.k3_rows);  // [nt][bits_words], after the transforms
+  const int tid = threadIdx.x, nth = blockDim.x;
   const int64_t blk = blockIdx.x;
-  const int tile = static_cast<int>(blk % tiles);
-  const int w = static_cast<int>((blk / tiles) % a.words);
-  const int64_t ld = blk / (static_cast<int64_t>(tiles) * a.words);
+  const int w = static_cast<int>(blk % a.words);
+  const int64_t ld = blk / a.words;
   const uint32_t drop = static_cast<uint32_t>(a.first_drop + ld);
-  const int nr = a.nr, nt = a.nt;
-  const int k0 = tile * kDetElements;
-  // B[col][k]: col < nt -> Re u_col: k < nr: Re A, else -Im A;  nt <= col < 2 nt -> Im u: Im A, Re A
-  const float2* am = a.det_a + ld * nt * nr;
-  for (int i = tid; i < 32 * kDetK; i += kDetThreads) {
-    const int col = i / kDetK, k = i - col * kDetK;
-    float v = 0.0f;
-    const int r = k < nr ? k : k - kMaxRx;
-    if (r >= 0 && r < nr && (k < nr || k >= kMaxRx)) {
-      if (col < nt) {
-        const float2 x = am[col * nr + r];
-        v = k < kMaxRx ? x.x : -x.y;
-      } else if (col < 2 * nt) {
-        const float2 x = am[(col - nt) * nr + r];
-        v = k < kMaxRx ? x.y : x.x;
-      }
-    }
-    const uint16_t h = f2bf(v);
-    bh[col * kDetKs + k] = h;
-    bl[col * kDetKs + k] = f2bf(v - bf2f(h));
+  const int n = a.n, nt = a.nt, nr = a.nr, rs = a.k3_rstride, half = a.n >> 1;
+  const double np = a.noise_power[ld];
+  const float rx_scale = static_cast<float>(1.0 / sqrt(static_cast<double>(n)));
+  const float sigma_scale = static_cast<float>(sqrt(0.5 * np) / sqrt(static_cast<double>(n)));
+  const float s2 = static_cast<float>(np);
+  fill_fft_pass_table(ptab, a.log2n);
+  const float2* da = a.det_a + ld * a.da_stride;
+  for (int e = tid; e < nr * kMaxUsers; e += nth) {
+    const int r = e / kMaxUsers, i = e - r * kMaxUsers;
+    am[e] = i < nt ? cscale(da[i * nr + r], sigma_scale) : make_float2(0.f, 0.f);
   }
-  __syncthreads();
-  {
-    // A operand straight from the grid: lane (m, kq) of wave v holds element k0 + 32 v + m and
-    // k = 4 kq .. 4 kq + 3 of each 8-wide step; step s < 8 takes the real parts of antennas
-    // 8 s + 4 kq + j, step s + 8 their imaginary parts, so one load of 4 complex samples feeds
-    // two steps.  Consecutive lanes read consecutive elements of one antenna row.
-    const int wave = tid >> 6, lane = tid & 63;
-    const int m = lane & 31, kq = lane >> 5;
-    const int el = k0 + 32 * wave + m;
-    const bool live = el < a.active;
-    const float2* gy = a.grid + (ld * a.words + w) * static_cast<int64_t>(nr) * a.active + (live ? el : 0);
-    floatx16 acc;
+  for (int e = tid; e < nt * nt; e += nth) {
+    bm[e] = cscale(da[nt * nr + e], rx_scale);
+    vm[e] = a.det_v[ld * nt * nt + e];
+  }
+  for (int i = tid; i < nt; i += nth) lam[i] = a.det_l[ld * nt + i];
+  if (tid == 0) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    const uint16_t* bhh = bh + m * kDetKs + 4 * kq;
-    const uint16_t* bll = bl + m * kDetKs + 4 * kq;
-#pragma unroll 2
-    for (int st = 0; st < kMaxRx / 8; ++st) {
-      float2 yv[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int r = 8 * st + 4 * kq + j;
-        yv[j] = live && r < nr ? gy[static_cast<int64_t>(r) * a.active] : make_float2(0.f, 0.f);
-      }
-      bf16x4 reh, rel, imh, iml;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint16_t hr = f2bf(yv[j].x), hi = f2bf(yv[j].y);
-        reh[j] = static_cast<short>(hr);
-        rel[j] = static_cast<short>(f2bf(yv[j].x - bf2f(hr)));
-        imh[j] = static_cast<short>(hi);
-        iml[j] = static_cast<short>(f2bf(yv[j].y - bf2f(hi)));
-      }
-      const int kr = 8 * st, ki = kMaxRx + 8 * st;
-      const bf16x4 brh = *reinterpret_cast<const bf16x4*>(bhh + kr);
-      const bf16x4 brl = *reinterpret_cast<const bf16x4*>(bll + kr);
-      const bf16x4 bih = *reinterpret_cast<const bf16x4*>(bhh + ki);
-      const bf16x4 bil = *reinterpret_cast<const bf16x4*>(bll + ki);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(rel, brh, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(reh, brl, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(reh, brh, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(iml, bih, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(imh, bil, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(imh, bih, acc, 0, 0, 0);
-    }
-    // D[row][col]: row = element 32 wave + (r / 4) 8 + 4 kq + r % 4, col = m
-    if (m < 2 * kMaxUsers) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) us[32 * wave + (r >> 2) * 8 + kq * 4 + (r & 3)][m] = acc[r];
+    for (int i = 0; i < 8; ++i) {
+      dmb->a[i] = a.dbounds.a[i];
+      dmb->b[i] = a.dbounds.b[i];
     }
   }
   __syncthreads();
-  // per element: x = V diag(conj(c) / (|c|^2 lambda + s2)) u, decisions, bit errors
-  int err = 0;
-  const float s2 = static_cast<float>(a.noise_power[ld]);
-  const float2* dv = a.det_v + ld * nt * nt;
-  const float* dl = a.det_l + ld * nt;
-  for (int e = tid; e < kDetElements; e += kDetThreads) {
-    const int k = k0 + e;
-    if (k >= a.active) continue;
-    const float2 c = a.csi[(ld * a.words + w) * static_cast<int64_t>(a.active) + k];
-    const float c2 = cabs2(c);
-    // fixed trip counts keep u in registers
-    float2 u[kMaxUsers];
+  // user rows at samples 2 j, 2 j + 1 of the data window, j = tid + p * blockDim
+  const int64_t win0 = static_cast<int64_t>(w) * (n + a.cp) + a.cp;
+  const float2* zd = a.z + (ld * nt * a.words + w) * static_cast<int64_t>(n);
+  float2 acc[kMaxUsers][2 * kLinkPairs];
+#pragma unroll
+  for (int i = 0; i < kMaxUsers; ++i)
+#pragma unroll
+    for (int q = 0; q < 2 * kLinkPairs; ++q) acc[i][q] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int p = 0; p < kLinkPairs; ++p) {
+    const int j = tid + p * nth;
+    if (j >= half) continue;
+    for (int t = 0; t < nt; ++t) {
+      const float4 zz = *reinterpret_cast<const float4*>(zd + static_cast<int64_t>(t) * a.words * n + 2 * j);
+#pragma unroll
+      for (int i = 0; i < kMaxUsers; ++i) {
+        if (i < nt) {
+          const float2 b = bm[i * nt + t];
+          acc[i][2 * p] = cfma(b, make_float2(zz.x, zz.y), acc[i][2 * p]);
+          acc[i][2 * p + 1] = cfma(b, make_float2(zz.z, zz.w), acc[i][2 * p + 1]);
+        }
+      }
+    }
+  }
+  // noise sample m of antenna r: words (0, 1) or (2, 3) of Philox call m / 2 of stream
+  // HPX_STREAM_NOISE + r; an even window start makes each pair one call
+  const bool aligned = (win0 & 1) == 0;
+  for (int r = 0; r < nr; ++r) {
+    float2 nz[2 * kLinkPairs];
+#pragma unroll
+    for (int p = 0; p < kLinkPairs; ++p) {
+      const int j = tid + p * nth;
+      nz[2 * p] = nz[2 * p + 1] = make_float2(0.f, 0.f);
+      if (j >= half) continue;
+      const int64_t m = win0 + 2 * j;
+      const uint4 wa = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>(m >> 1));
+      if (aligned) {
+        nz[2 * p] = complex_normal_f32(wa.x, wa.y);
+        nz[2 * p + 1] = complex_normal_f32(wa.z, wa.w);
+      } else {
+        const uint4 wb = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>((m + 1) >> 1));
+        nz[2 * p] = complex_normal_f32(wa.z, wa.w);
+        nz[2 * p + 1] = complex_normal_f32(wb.x, wb.y);
+      }
+    }
+    const float2* ar = am + r * kMaxUsers;
 #pragma unroll
     for (int i = 0; i < kMaxUsers; ++i) {
-      u[i] = make_float2(0.f, 0.f);
       if (i < nt) {
-        const float den = fmaf(c2, dl[i], s2);
-        const float gi = (dl[i] > 0.0f || s2 > 0.0f) && den > 0.0f ? 1.0f / den : 0.0f;
-        u[i] = cscale(cmulc(c, make_float2(us[e][i], us[e][nt + i])), gi);
+        const float2 c = ar[i];
+#pragma unroll
+        for (int q = 0; q < 2 * kLinkPairs; ++q) acc[i][q] = cfma(c, nz[q], acc[i][q]);
       }
     }
-    const int64_t sym = static_cast<int64_t>(w) * a.active + k;
+  }
+  // transforms: rows v0 .. v0 + kLinkRows - 1 of (users 0 .. nt - 1, the CSI row) per round
+  float2 u[kMaxUsers][kLinkBins];
+  float2 c[kLinkBins];
+#pragma unroll
+  for (int b = 0; b < kLinkBins; ++b) {
+    c[b] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < kMaxUsers; ++i) u[i][b] = make_float2(0.f, 0.f);
+  }
+  // one round per compile-time index, so that acc, u and c stay in registers
+  auto round = [&](auto qc) {
+    constexpr int v0 = decltype(qc)::value * kLinkRows;
+    if (v0 > nt) return;
+    const int cnt = min(kLinkRows, nt + 1 - v0);
+    const bool csi_here = nt < v0 + kLinkRows;
+#pragma unroll
+    for (int i = v0; i < v0 + kLinkRows && i < kMaxUsers; ++i) {
+      if (i < nt) {
+        float2* row = rows + (i - v0) * rs;
+#pragma unroll
+        for (int p = 0; p < kLinkPairs; ++p) {
+          const int j = tid + p * nth;
+          if (j < half)
+            *reinterpret_cast<float4*>(row + 2 * j) =
+                make_float4(acc[i][2 * p].x, acc[i][2 * p].y, acc[i][2 * p + 1].x, acc[i][2 * p + 1].y);
+        }
+      }
+    }
+    if (csi_here) {
+      float2* row = rows + (nt - v0) * rs;
+      for (int k = tid; k < n; k += nth) row[k] = make_float2(0.f, 0.f);
+      __syncthreads();
+      if (tid == 0) {
+        const float2* ab = a.abar + ld * a.paths * a.words;
+        for (int p = 0; p < a.paths; ++p) {
+          const int d = a.delays[p];
+          if (d >= 0 && d < a.keep_below) row[d] = cadd(row[d], ab[p * a.words + w]);
+        }
+      }
+    }
+    __syncthreads();
+    lds_fft_padded<-1>(rows, cnt, rs, a.log2n, ptab);
+#pragma unroll
+    for (int b = 0; b < kLinkBins; ++b) {
+      const int k = tid + b * nth;
+      if (k >= a.active) continue;
+      const int bin = subcarrier_bin(a.active_lo + k, n, a.dc);
+#pragma unroll
+      for (int i = v0; i < v0 + kLinkRows && i < kMaxUsers; ++i)
+        if (i < nt) u[i][b] = rows[(i - v0) * rs + bin];
+      if (csi_here) c[b] = rows[(nt - v0) * rs + bin];
+    }
+    __syncthreads();
+  };
+  static_assert((kMaxUsers + 1 + kLinkRows - 1) / kLinkRows == 3, "three transform rounds");
+  round(std::integral_constant<int, 0>{});
+  round(std::integral_constant<int, 1>{});
+  round(std::integral_constant<int, 2>{});
+  // transmitted bits of word w of every user (streams HPX_STREAM_BITS + t), over the rows
+  const int64_t bit0 = static_cast<int64_t>(w) * a.active * a.bps;
+  const int bo = static_cast<int>(bit0 & 31);
+  const int fw = static_cast<int>(bit0 >> 5);
+  const int lw = static_cast<int>((bit0 + static_cast<int64_t>(a.active) * a.bps - 1) >> 5);
+  const int calls = (lw >> 2) - (fw >> 2) + 1;
+  const int bw = a.bits_words;
+  for (int e = tid; e < nt * calls; e += nth) {
+    const int t = e / calls, cc = (fw >> 2) + (e - t * calls);
+    const uint4 r = stream_call(a.seed, drop, HPX_STREAM_BITS + t, static_cast<uint32_t>(cc));
+    const uint32_t comp[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int wi = 4 * cc + k - fw;
+      if (wi >= 0 && wi <= lw - fw) bits[t * bw + wi] = comp[k];
+    }
+  }
+  __syncthreads();
+  // per subcarrier: x = V diag(conj(c) / (|c|^2 lambda + s2)) u, decisions, bit errors
+  int err = 0;
+#pragma unroll
+  for (int b = 0; b < kLinkBins; ++b) {
+    const int k = tid + b * nth;
+    if (k >= a.active) continue;
+    const float c2 = cabs2(c[b]);
+    float2 g[kMaxUsers];
+#pragma unroll
+    for (int i = 0; i < kMaxUsers; ++i) {
+      g[i] = make_float2(0.f, 0.f);
+      if (i < nt) {
+        const float den = fmaf(c2, lam[i], s2);
+        const float gi = (lam[i] > 0.0f || s2 > 0.0f) && den > 0.0f ? 1.0f / den : 0.0f;
+        g[i] = cscale(cmulc(c[b], u[i][b]), gi);
+      }
+    }
+    const int sbit = bo + k * a.bps;
     for (int t = 0; t < nt; ++t) {
       float2 xh = make_float2(0.f, 0.f);
 #pragma unroll
       for (int i = 0; i < kMaxUsers; ++i)
-        if (i < nt) xh = cfma(dv[t * nt + i], u[i], xh);
-      const uint32_t rx = demap_symbol(a.mod, static_cast<double>(xh.x), static_cast<double>(xh.y));
-      // transmitted bits of symbol sym of user t
-      const int64_t pos = sym * a.bps;
-      const int64_t wi = pos >> 5;
-      const uint4 q = stream_call(a.seed, drop, HPX_STREAM_BITS + t, static_cast<uint32_t>(wi >> 2));
-      const uint32_t comp[4] = {q.x, q.y, q.z, q.w};
-      uint64_t v = comp[wi & 3];
-      const int sh = static_cast<int>(pos & 31);
-      if (sh + a.bps > 32) {
-        const int64_t wn = wi + 1;
-        const uint32_t nx = (wn & 3) ? comp[wn & 3]
-                                     : stream_call(a.seed, drop, HPX_STREAM_BITS + t, static_cast<uint32_t>(wn >> 2)).x;
-        v |= static_cast<uint64_t>(nx) << 32;
-      }
-      const uint32_t tx = static_cast<uint32_t>(v >> sh) & ((1u << a.bps) - 1u);
+        if (i < nt) xh = cfma(vm[t * nt + i], g[i], xh);
+      const uint32_t rx = demap_symbol_f32(a.mod, dmb, xh.x, xh.y);
+      const uint32_t tx = symbol_bits_at(bits + t * bw, sbit, a.bps);
       err += __popc(rx ^ tx);
       if (a.equalized) a.equalized[((ld * nt + t) * a.words + w) * static_cast<int64_t>(a.active) + k] = xh;
     }
@@ -568,7 +656,7 @@ __global__ void __launch_bounds__(kDetThreads) massive_detect_kernel(MassiveArgs
   __syncthreads();
   if (tid == 0) {
     int s = 0;
-    for (int i = 0; i < kDetThreads / 64; ++i) s += red[i];
+    for (int i = 0; i < (nth >> 6); ++i) s += red[i];
     if (s) atomicAdd(a.errors + ld, s);
   }
 }
@@ -664,11 +752,36 @@ int make_massive(const hpx_massive_link* l, MassiveArgs* out) {
   a.k2_y = 0;
   a.k2_tw = static_cast<int>(align16(static_cast<int64_t>(kRxRows + 1) * a.n * 8));
   a.k2_lds = static_cast<int>(a.k2_tw + align16(static_cast<int64_t>(a.n) * 8));
-  if (a.k0_lds > 160 * 1024 || a.k1_lds > 160 * 1024 || a.k2_lds > 160 * 1024) {
+  // K3 LDS: the transform rows (the padded extent N + N / 32 each; later the word's bits), the
+  // pass table, B, sigma A^T, V, lambda, the demapper's boundaries, the error partials
+  a.da_stride = a.nt * a.nr + a.nt * a.nt;
+  a.dbounds = demap_bounds(a.mod);
+  a.k3_rstride = a.n + (a.n >> kFftPadShift);
+  o = 0;
+  a.k3_rows = 0;
+  o += align16(std::max<int64_t>(static_cast<int64_t>(kLinkRows) * a.k3_rstride * 8,
+                                 static_cast<int64_t>(a.nt) * a.bits_words * 4));
+  a.k3_tab = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(fft_pass_table_size(a.n)) * 8);
+  a.k3_b = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.nt) * a.nt * 8);
+  a.k3_a = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.nr) * kMaxUsers * 8);
+  a.k3_v = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.nt) * a.nt * 8);
+  a.k3_l = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(a.nt) * 4);
+  a.k3_dmb = static_cast<int>(o);
+  o += align16(sizeof(DemapBounds));
+  a.k3_red = static_cast<int>(o);
+  o += align16((kLinkThreads / 64) * 4);
+  a.k3_lds = static_cast<int>(o);
+  if (a.k0_lds > 160 * 1024 || a.k1_lds > 160 * 1024 || a.k2_lds > 160 * 1024 || a.k3_lds > 160 * 1024) {
     set_error("hpx_massive_run: the link needs more than 160 KB of LDS per workgroup");
     return HPX_ERR_UNSUPPORTED;
   }
-  if (a.n > kFftMaxPerThread * kRxThreads) {
+  if (a.n > kFftMaxPerThread * kRxThreads || a.n > 2 * kLinkPairs * kLinkThreads ||
+      a.active > kLinkBins * kLinkThreads) {
     set_error("hpx_massive_run: %d subcarriers exceed one workgroup's FFT capacity", a.n);
     return HPX_ERR_UNSUPPORTED;
   }
@@ -691,7 +804,7 @@ extern "C" int hpx_massive_sizes(const hpx_massive_link* link_host, int64_t* siz
   sizes_host[1] = static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs;     // coef
   sizes_host[2] = static_cast<int64_t>(a.paths) * a.words;                  // abar
   sizes_host[3] = static_cast<int64_t>(a.nt) * a.nt;                        // det_v
-  sizes_host[4] = static_cast<int64_t>(a.nt) * a.nr;                        // det_a
+  sizes_host[4] = a.da_stride;                                              // det_a (A, then B)
   sizes_host[5] = a.nt;                                                     // det_l (float32)
   sizes_host[6] = static_cast<int64_t>(a.nt) * a.words * a.n;               // z
   sizes_host[7] = static_cast<int64_t>(a.words) * a.nr * a.active;          // grid
@@ -711,8 +824,9 @@ extern "C" int hpx_massive_run(const hpx_massive_link* link_host, const int32_t*
   if (num_drops == 0) return HPX_OK;
   HPX_REQUIRE(delays && path_gains && noise_power && work_host && errors, "hpx_massive_run: null pointer");
   const hpx_massive_work& wk = *work_host;
-  HPX_REQUIRE(wk.spatial && wk.coef && wk.abar && wk.det_v && wk.det_a && wk.det_l && wk.z && wk.grid && wk.csi,
+  HPX_REQUIRE(wk.spatial && wk.coef && wk.abar && wk.det_v && wk.det_a && wk.det_l && wk.z,
               "hpx_massive_run: null work buffer");
+  HPX_REQUIRE((wk.grid == nullptr) == (wk.csi == nullptr), "hpx_massive_run: grid and csi are both given or both null");
   a.delays = delays;
   a.path_gains = path_gains;
   a.corr_rx = reinterpret_cast<const double*>(corr_rx);
@@ -734,19 +848,21 @@ extern "C" int hpx_massive_run(const hpx_massive_link* link_host, const int32_t*
   const hipStream_t s = as_stream(stream);
   if (!raise_lds(reinterpret_cast<const void*>(massive_prologue_kernel), a.k0_lds) ||
       !raise_lds(reinterpret_cast<const void*>(massive_tx_kernel), a.k1_lds) ||
-      !raise_lds(reinterpret_cast<const void*>(massive_rx_kernel), a.k2_lds)) {
+      !raise_lds(reinterpret_cast<const void*>(massive_rx_kernel), a.k2_lds) ||
+      !raise_lds(reinterpret_cast<const void*>(massive_link_kernel), a.k3_lds)) {
     set_error("hpx_massive_run: LDS limit not available");
     return HPX_ERR_UNSUPPORTED;
   }
   const unsigned nd = static_cast<unsigned>(num_drops);
   hipLaunchKernelGGL(massive_prologue_kernel, dim3(nd), dim3(256), a.k0_lds, s, a);
   hipLaunchKernelGGL(massive_tx_kernel, dim3(nd * a.nt * a.words), dim3(kTxThreads), a.k1_lds, s, a);
-  const int64_t groups = (a.nr + kRxGroup - 1) / kRxGroup;
-  a.num_pairs = static_cast<int64_t>(nd) * a.words;
-  const int64_t pair_sets = (a.num_pairs + kXcds - 1) / kXcds;
-  hipLaunchKernelGGL(massive_rx_kernel, dim3(static_cast<unsigned>(pair_sets * kXcds * groups)), dim3(kRxThreads),
-                     a.k2_lds, s, a);
-  const unsigned tiles = static_cast<unsigned>((a.active + kDetElements - 1) / kDetElements);
-  hipLaunchKernelGGL(massive_detect_kernel, dim3(nd * a.words * tiles), dim3(kDetThreads), 0, s, a);
+  if (a.grid) {  // the per-antenna grid and the CSI factor, on request
+    const int64_t groups = (a.nr + kRxGroup - 1) / kRxGroup;
+    a.num_pairs = static_cast<int64_t>(nd) * a.words;
+    const int64_t pair_sets = (a.num_pairs + kXcds - 1) / kXcds;
+    hipLaunchKernelGGL(massive_rx_kernel, dim3(static_cast<unsigned>(pair_sets * kXcds * groups)), dim3(kRxThreads),
+                       a.k2_lds, s, a);
+  }
+  hipLaunchKernelGGL(massive_link_kernel, dim3(nd * a.words), dim3(kLinkThreads), a.k3_lds, s, a);
   return check_launch("hpx_massive_run");
 }

@@ -5,8 +5,10 @@ LMMSE detection on ideal CSI.
 
 Beyond the reference's 10-antenna arrays (``hermespy/channel/fading/fading.py:476-489``);
 ``hpx_massive_run`` documents how the spatial draw is extended.  A drop does not fit one
-CU, so it runs as four kernels over per-drop device buffers held here (prologue,
-transmit + delay line, receive FFTs, MFMA detection); see ``hermespy_amd/csrc/massive.hip``.
+CU, so it runs as three kernels over per-drop device buffers held here (prologue, transmit +
+delay line, receive + detection per (drop, word)); the receiver forms the matched-filter
+outputs of the 64 antennas directly, one transform per user, and writes the per-antenna grid
+only on request (a fourth kernel); see ``hermespy_amd/csrc/massive.hip``.
 """
 
 from __future__ import annotations
@@ -83,15 +85,23 @@ class MassiveConfig:
 
 
 WORK_FIELDS = ("spatial", "coef", "abar", "det_v", "det_a", "det_l", "z", "grid", "csi")
+EXPORT_FIELDS = ("grid", "csi")
 
 
 class MassiveBatch:
-    """Device tables and per-drop buffers for launches of up to `chunk` drops."""
+    """Device tables and per-drop buffers for launches of up to `chunk` drops.
 
-    def __init__(self, cfg: MassiveConfig, chunk: int = 32, device: str | torch.device = "cuda") -> None:
+    The received per-antenna grid (``buffers["grid"]``, [drop][words][nr][active]) and the
+    ideal-CSI factor (``buffers["csi"]``) are diagnostic exports: with ``keep_grid=True`` every
+    run writes them, with ``keep_grid=None`` (default) the runs that export equalized symbols
+    do, and with ``False`` none.  They are allocated on first use."""
+
+    def __init__(self, cfg: MassiveConfig, chunk: int = 32, device: str | torch.device = "cuda",
+                 keep_grid: bool | None = None) -> None:
         self.cfg = cfg
         self.chunk = int(chunk)
         self.device = torch.device(device)
+        self.keep_grid = keep_grid
         lib = _lib.load()
         self.c_link = cfg.c_link()
         sizes = np.zeros(9, dtype=np.int64)
@@ -100,13 +110,22 @@ class MassiveBatch:
         dev = self.device
         self.buffers = {}
         for name, count in zip(WORK_FIELDS, sizes):
+            if name in EXPORT_FIELDS:
+                continue
             dt = torch.float32 if name == "det_l" else torch.complex64
             self.buffers[name] = torch.empty((self.chunk, int(count)), dtype=dt, device=dev)
-        self.work = _lib.HpxMassiveWork(*[self.buffers[n].data_ptr() for n in WORK_FIELDS])
         lk = cfg.link()
         self.delays = torch.as_tensor(lk.delay_samples, dtype=torch.int32, device=dev)
         self.path_gains = torch.as_tensor(lk.path_gains, dtype=torch.float64, device=dev).contiguous()
         self.corr_rx = torch.as_tensor(np.ascontiguousarray(cfg.corr_rx), device=dev)
+
+    def _work(self, export: bool) -> _lib.HpxMassiveWork:
+        if export and "grid" not in self.buffers:
+            for name in EXPORT_FIELDS:
+                count = int(self.sizes[WORK_FIELDS.index(name)])
+                self.buffers[name] = torch.empty((self.chunk, count), dtype=torch.complex64, device=self.device)
+        return _lib.HpxMassiveWork(*[self.buffers[n].data_ptr() if (export or n not in EXPORT_FIELDS) else None
+                                     for n in WORK_FIELDS])
 
     def run(self, seed: int, first_drop: int, noise_power: torch.Tensor, errors: torch.Tensor,
             equalized: torch.Tensor | None = None) -> None:
@@ -123,7 +142,9 @@ class MassiveBatch:
         if equalized is not None and (equalized.dtype != torch.complex64 or
                                       equalized.numel() < n * c.num_users * c.words * c.active):
             raise ValueError("equalized must be complex64 [drops][users][words][active]")
+        export = self.keep_grid if self.keep_grid is not None else equalized is not None
+        work = self._work(bool(export))
         _lib.check(_lib.load().hpx_massive_run(
             ctypes.byref(self.c_link), _lib.ptr(self.delays), _lib.ptr(self.path_gains), _lib.ptr(self.corr_rx), None,
-            ctypes.c_uint64(seed & (2**64 - 1)), int(first_drop), n, _lib.ptr(noise_power), ctypes.byref(self.work),
+            ctypes.c_uint64(seed & (2**64 - 1)), int(first_drop), n, _lib.ptr(noise_power), ctypes.byref(work),
             _lib.ptr(errors), _lib.ptr(equalized), _lib.stream_handle()))

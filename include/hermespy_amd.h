@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define HPX_ABI_VERSION 6
+#define HPX_ABI_VERSION 7
 
 typedef enum hpx_status {
   HPX_OK = 0,
@@ -510,11 +510,11 @@ typedef struct hpx_massive_work {
   hpx_c64* coef;     /* [paths][blocks][5] Taylor coefficients of the taps */
   hpx_c64* abar;     /* [paths][words] tap means */
   hpx_c64* det_v;    /* [users][users] */
-  hpx_c64* det_a;    /* [users][nr] V^H S^H */
+  hpx_c64* det_a;    /* [users][nr] A = V^H S^H, then [users][users] B = A S */
   float* det_l;      /* [users] eigenvalues of S^H S */
   hpx_c64* z;        /* [users][words][N] delay-line outputs over the data windows */
-  hpx_c64* grid;     /* [words][nr][active] received active subcarriers */
-  hpx_c64* csi;      /* [words][active] ideal-CSI factor c (H = S c) */
+  hpx_c64* grid;     /* [words][nr][active] received active subcarriers, or NULL */
+  hpx_c64* csi;      /* [words][active] ideal-CSI factor c (H = S c), NULL with grid */
 } hpx_massive_work;
 
 /* Per-drop element counts of the nine hpx_massive_work buffers. */
@@ -522,7 +522,9 @@ int hpx_massive_sizes(const hpx_massive_link* link_host, int64_t* sizes_host);
 
 /* num_drops (<= 65535) drops first_drop.. with noise_power[d]; errors[d] += the drop's bit errors
  * (zero them first); equalized [drops][users][words][active] or NULL.  delays / path_gains /
- * corr_rx / corr_tx as in hpx_run_batch (corr_tx may be NULL with corr_rx given: identity). */
+ * corr_rx / corr_tx as in hpx_run_batch (corr_tx may be NULL with corr_rx given: identity).
+ * The detection never needs the per-antenna grid: work->grid and work->csi are exports, written
+ * when given (both or neither). */
 int hpx_massive_run(const hpx_massive_link* link_host, const int32_t* delays, const double* path_gains,
                     const hpx_c128* corr_rx, const hpx_c128* corr_tx, uint64_t seed, int64_t first_drop,
                     int64_t num_drops, const double* noise_power, const hpx_massive_work* work_host, int32_t* errors,
