@@ -7,8 +7,9 @@
 //                                           means, and the per-drop detector V, lambda of S^H S,
 //                                           A = V^H S^H and B = A S (fading.py:468-515,
 //                                           channel_estimation.py:215-295)
-//   K1 transmit  (workgroup per drop, user, word) bits -> 256-QAM -> OFDM + CP -> the user's tapped
-//                                           delay line over the word's data window (modem.py:797-888,
+//   K1 transmit  (workgroup per drop and user pair, stepping through the words)
+//                                           bits -> 256-QAM -> OFDM + CP -> the user's tapped delay
+//                                           line over each word's data window (modem.py:797-888,
 //                                           ofdm.py:97-130, fading.py:371-406)
 //   K3 receive + detect (workgroup per drop, word)
 //                                           the matched-filter outputs u = A y of every active
@@ -50,11 +51,14 @@ namespace {
 
 constexpr int kMaxRx = 64, kMaxUsers = 8;
 constexpr int kTxThreads = 1024, kRxThreads = 1024;
+constexpr int kTxUsers = 2;      // users per transmit workgroup (they share the tap evaluations)
+constexpr int kTxSamples = 4;    // data-window samples per thread of K1 (N <= 4 kTxThreads)
 constexpr int kRxGroup = 8;      // antennas per receive workgroup
 constexpr int kRxRows = 2;       // antennas transformed together
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kXcds = 8;         // MI355X dispatches consecutive workgroups round robin over 8 XCDs
 constexpr int kLinkThreads = 1024;
-constexpr int kLinkPairs = 2;    // sample pairs per thread of K3 (N <= 4 kLinkThreads)
+constexpr int kLinkPasses = 8;   // 16-pair column tiles per wave of K3 (N <= 2 * 16 * 8 * 16 waves)
 constexpr int kLinkBins = 4;     // active subcarriers per thread of K3 (active <= 4 kLinkThreads)
 constexpr int kLinkRows = 3;     // rows per transform round of K3 (users, then the CSI row)
 
@@ -88,9 +92,9 @@ struct MassiveArgs {
   int32_t* errors;  // [drop], accumulated
   float2* equalized;  // [drop][nt][words][active] or null
   int k0_uni, k0_sos, k0_e, k0_f, k0_s64, k0_det, k0_lds;
-  int k1_x, k1_tw, k1_coef, k1_bits, k1_cons, k1_dly, k1_lds;
+  int k1_x, k1_tab, k1_coef, k1_bits, k1_cons, k1_dly, k1_lds;
   int k2_y, k2_tw, k2_lds;
-  int k3_rows, k3_rstride, k3_tab, k3_b, k3_a, k3_v, k3_l, k3_dmb, k3_red, k3_lds;
+  int k3_rows, k3_rstride, k3_tab, k3_atab, k3_v, k3_l, k3_dmb, k3_red, k3_lds;
   DemapBounds dbounds;  // float32 decision boundaries of mod (exact hard decisions)
 };
 
@@ -260,89 +264,134 @@ __global__ void __launch_bounds__(256) massive_prologue_kernel(MassiveArgs a) {
 }
 
 // ---------------------------------------------------------------- K1: transmitter + delay line
-// One workgroup per (drop, user, word).  The delay line of word w reaches dx samples back into
-// word w - 1, so the workgroup first rebuilds word w - 1's symbol (bits, mapping, IFFT) for that
-// tail, then word w's: twice the symbol work for 14 times the workgroups of a per-user loop.
-
-// bits, mapping and IFFT of word w of user t into x[x0 .. x0 + n) (ends with a barrier)
-__device__ void massive_tx_symbol(const MassiveArgs& a, uint32_t drop, int t, int w, float2* x, const float2* tw,
-                                  uint32_t* bits, const float2* cons) {
-  const int tid = threadIdx.x, nth = blockDim.x, n = a.n;
-  const int64_t bit0 = static_cast<int64_t>(w) * a.active * a.bps;
-  const int bo = static_cast<int>(bit0 & 31);
-  const int fw = static_cast<int>(bit0 >> 5);
-  const int lw = static_cast<int>((bit0 + static_cast<int64_t>(a.active) * a.bps - 1) >> 5);
-  for (int c = (fw >> 2) + tid; c <= (lw >> 2); c += nth) {
-    const uint4 r = stream_call(a.seed, drop, HPX_STREAM_BITS + t, static_cast<uint32_t>(c));
-    const uint32_t comp[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int wi = 4 * c + k - fw;
-      if (wi >= 0 && wi <= lw - fw) bits[wi] = comp[k];
-    }
-  }
-  __syncthreads();
-  for (int q = tid; q < n; q += nth) {
-    const int el = q - a.active_lo;
-    x[a.x0 + subcarrier_bin(q, n, a.dc)] =
-        (el >= 0 && el < a.active) ? cons[symbol_bits_at(bits, bo + el * a.bps, a.bps)] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  lds_fft<+1>(x + a.x0, 1, a.xs, a.log2n, tw);
-}
-
+// One workgroup per (drop, kTxUsers users), stepping through the words: bits, mapping and IFFT of
+// word w of each user, the cyclic prefix, then the tapped delay line over the word's data window,
+// with each tap value evaluated once for the group's users.  The delay line reaches dx samples
+// back into word w - 1; those samples stay in LDS from the previous step (silence before the
+// frame).  Per user, x holds [word w - 1's last dx samples | prefix | word w], the word in the
+// FFT's padded extent.
 __global__ void __launch_bounds__(kTxThreads) massive_tx_kernel(MassiveArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
-  float2* x = reinterpret_cast<float2*>(lds + a.k1_x);
-  float2* tw = reinterpret_cast<float2*>(lds + a.k1_tw);
-  float2* coef = reinterpret_cast<float2*>(lds + a.k1_coef);
-  uint32_t* bits = reinterpret_cast<uint32_t*>(lds + a.k1_bits);
+  float2* x = reinterpret_cast<float2*>(lds + a.k1_x);        // [kTxUsers][xs]
+  float2* ptab = reinterpret_cast<float2*>(lds + a.k1_tab);   // FFT pass table
+  float2* coef = reinterpret_cast<float2*>(lds + a.k1_coef);  // [L][nb][5]
+  uint32_t* bits = reinterpret_cast<uint32_t*>(lds + a.k1_bits);  // [kTxUsers][bits_words]
   float2* cons = reinterpret_cast<float2*>(lds + a.k1_cons);
-  int* dly = reinterpret_cast<int*>(lds + a.k1_dly);  // path delays: LDS broadcasts in the delay line
+  int* dly = reinterpret_cast<int*>(lds + a.k1_dly);  // path delays
   const int tid = threadIdx.x, nth = blockDim.x;
-  const int64_t blk = blockIdx.x;
-  const int w = static_cast<int>(blk % a.words);
-  const int64_t lt = blk / a.words;
-  const int64_t ld = lt / a.nt;
-  const int t = static_cast<int>(lt - ld * a.nt);
+  const int groups = (a.nt + kTxUsers - 1) / kTxUsers;
+  const int64_t ld = blockIdx.x / groups;
+  const int t0 = static_cast<int>(blockIdx.x % groups) * kTxUsers;
+  const int nu = min(kTxUsers, a.nt - t0);
   const uint32_t drop = static_cast<uint32_t>(a.first_drop + ld);
-  const int n = a.n, L = a.paths, x0 = a.x0;
+  const int n = a.n, L = a.paths, x0 = a.x0, xs = a.xs, bw = a.bits_words;
   const int ncoef = L * a.nb * kTaylorCoeffs;
+  const int lane = tid & 63;
+  const int wbase = __builtin_amdgcn_readfirstlane(tid);  // the wavefront's first thread
   for (int i = tid; i < ncoef; i += nth) coef[i] = a.coef[ld * ncoef + i];
   for (int p = tid; p < L; p += nth) dly[p] = a.delays[p];
   for (int v = tid; v < a.order; v += nth) {
     const double2 p = constellation_point(a.order, 1, bits_to_label(static_cast<uint32_t>(v), a.bps));
     cons[v] = make_float2(static_cast<float>(p.x * a.tx_scale), static_cast<float>(p.y * a.tx_scale));
   }
-  fill_twiddles_octants(tw, n);
-  __syncthreads();
-  // the last dx data samples of word w - 1 (silence before the frame)
-  if (w > 0) {
-    massive_tx_symbol(a, drop, t, w - 1, x, tw, bits, cons);
-    for (int i = tid; i < a.dx; i += nth) x[i] = x[x0 + n - a.dx + i];
-  } else {
-    for (int i = tid; i < a.dx; i += nth) x[i] = make_float2(0.f, 0.f);
+  for (int e = tid; e < kTxUsers * a.dx; e += nth) {
+    const int u = e / a.dx;
+    x[u * xs + (e - u * a.dx)] = make_float2(0.f, 0.f);
   }
-  __syncthreads();
-  massive_tx_symbol(a, drop, t, w, x, tw, bits, cons);
-  for (int i = tid; i < a.cpl; i += nth) x[x0 - a.cpl + i] = x[x0 + n - a.cpl + i];
-  __syncthreads();
-  // tapped delay line over the data window (the receiver keeps nothing else)
+  fill_fft_pass_table(ptab, a.log2n);
   const TaylorBlocks tb{a.log2b};
-  const int64_t win0 = static_cast<int64_t>(w) * (n + a.cp) + a.cp;
-  float2* zt = a.z + (ld * a.nt + t) * static_cast<int64_t>(a.words) * n + static_cast<int64_t>(w) * n;
-  for (int nl = tid; nl < n; nl += nth) {
-    const int64_t nabs = win0 + nl;
-    float2 acc = make_float2(0.f, 0.f);
-    for (int p = 0; p < L; ++p) {
-      const int d = dly[p];
-      const int64_t m = nabs - d;
-      if (d < 0 || d > a.maxd || m < 0) continue;
-      const float2 h = taylor_eval(coef + (p * a.nb + static_cast<int>(tb.block_of(m))) * kTaylorCoeffs,
-                                   tb.offset(m));
-      acc = cfma(h, x[x0 + nl - d], acc);
+  for (int w = 0; w < a.words; ++w) {
+    // bits of word w of each user (streams HPX_STREAM_BITS + t)
+    const int64_t bit0 = static_cast<int64_t>(w) * a.active * a.bps;
+    const int bo = static_cast<int>(bit0 & 31);
+    const int fw = static_cast<int>(bit0 >> 5);
+    const int lw = static_cast<int>((bit0 + static_cast<int64_t>(a.active) * a.bps - 1) >> 5);
+    const int calls = (lw >> 2) - (fw >> 2) + 1;
+    for (int e = tid; e < nu * calls; e += nth) {
+      const int u = e / calls, c = (fw >> 2) + (e - u * calls);
+      const uint4 r = stream_call(a.seed, drop, HPX_STREAM_BITS + t0 + u, static_cast<uint32_t>(c));
+      const uint32_t comp[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int wi = 4 * c + k - fw;
+        if (wi >= 0 && wi <= lw - fw) bits[u * bw + wi] = comp[k];
+      }
     }
-    zt[nl] = acc;
+    __syncthreads();  // also orders the previous step's tail copies before the mapping
+    for (int e = tid; e < nu * n; e += nth) {
+      const int u = e >> a.log2n, q = e & (n - 1);
+      const int el = q - a.active_lo;
+      x[u * xs + x0 + subcarrier_bin(q, n, a.dc)] =
+          (el >= 0 && el < a.active) ? cons[symbol_bits_at(bits + u * bw, bo + el * a.bps, a.bps)]
+                                     : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    lds_fft_padded<+1>(x + x0, nu, xs, a.log2n, ptab);
+    for (int e = tid; e < nu * a.cpl; e += nth) {
+      const int u = e / a.cpl, i = e - u * a.cpl;
+      x[u * xs + x0 - a.cpl + i] = x[u * xs + x0 + n - a.cpl + i];
+    }
+    __syncthreads();
+    // tapped delay line over the data window (the receiver keeps nothing else)
+    const int win0 = w * (n + a.cp) + a.cp;
+    // Wavefront v filters samples 64 kTxSamples v + 64 s + lane (s < kTxSamples): when they all
+    // lie in one Taylor block (blocks hold >= 64 samples), one broadcast read of the block's
+    // coefficients serves the wavefront's kTxSamples samples; otherwise each lane reads its own.
+    // All kTxUsers rows are filtered (a missing user's row is never stored).
+    float2 acc[kTxUsers][kTxSamples];
+#pragma unroll
+    for (int u = 0; u < kTxUsers; ++u)
+#pragma unroll
+      for (int s = 0; s < kTxSamples; ++s) acc[u][s] = make_float2(0.f, 0.f);
+    const int nlw = wbase * kTxSamples;  // the wavefront's first sample (wave-uniform)
+    if (nlw < n) {
+      for (int p = 0; p < L; ++p) {
+        const int d = dly[p];
+        if (d < 0 || d > a.maxd) continue;
+        const int mw = win0 + nlw - d;
+        const int mlast = mw + min(64 * kTxSamples, n - nlw) - 1;
+        const float2* cp = coef + p * a.nb * kTaylorCoeffs;
+        if (mw >= 0 && (mw >> a.log2b) == (mlast >> a.log2b)) {
+          const float2* cb = cp + (mw >> a.log2b) * kTaylorCoeffs;
+          float2 cc[kTaylorCoeffs];
+#pragma unroll
+          for (int k = 0; k < kTaylorCoeffs; ++k) cc[k] = cb[k];
+#pragma unroll
+          for (int s = 0; s < kTxSamples; ++s) {
+            const int nl = nlw + 64 * s + lane;
+            const float2 h = taylor_eval(cc, tb.offset32(mw + 64 * s + lane));
+#pragma unroll
+            for (int u = 0; u < kTxUsers; ++u) acc[u][s] = cfma(h, x[u * xs + x0 + nl - d], acc[u][s]);
+          }
+        } else {
+#pragma unroll
+          for (int s = 0; s < kTxSamples; ++s) {
+            const int nl = nlw + 64 * s + lane;
+            const int m = mw + 64 * s + lane;
+            if (nl >= n || m < 0) continue;
+            const float2 h = taylor_eval(cp + (m >> a.log2b) * kTaylorCoeffs, tb.offset32(m));
+#pragma unroll
+            for (int u = 0; u < kTxUsers; ++u) acc[u][s] = cfma(h, x[u * xs + x0 + nl - d], acc[u][s]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kTxUsers; ++u) {
+      if (u >= nu) continue;
+      float2* zt = a.z + ((ld * a.nt + t0 + u) * a.words + w) * static_cast<int64_t>(n);
+#pragma unroll
+      for (int s = 0; s < kTxSamples; ++s) {
+        const int nl = nlw + 64 * s + lane;
+        if (nl < n) zt[nl] = acc[u][s];
+      }
+    }
+    __syncthreads();
+    // word w's last dx samples become the next word's tail
+    for (int e = tid; e < nu * a.dx; e += nth) {
+      const int u = e / a.dx, i = e - u * a.dx;
+      x[u * xs + i] = x[u * xs + x0 + n - a.dx + i];
+    }
   }
 }
 
@@ -444,19 +493,24 @@ __global__ void __launch_bounds__(kRxThreads) massive_rx_kernel(MassiveArgs a) {
 }
 
 // ---------------------------------------------------------------- K3: receive + detection
-// One workgroup per (drop, word).  Each thread forms the user rows
-//   row_i = sum_t B[i][t] z_t + sigma sum_r A[i][r] n_r          (B = A S, scaled by 1 / sqrt(N))
-// at its kLinkPairs sample pairs in registers, drawing every antenna's noise (Philox stream
-// HPX_STREAM_NOISE + r, the samples massive_rx_kernel adds) and folding it in at once.  The rows
-// and the ideal-CSI row (tap means at the path delays) are then transformed kLinkRows at a time in
-// LDS; each thread keeps the active subcarriers tid + b * blockDim (b < kLinkBins) of every row in
-// registers and detects them after the last round.
+// One workgroup per (drop, word).  The user rows
+//   row_i = sum_t B[i][t] z_t + sigma sum_r A[i][r] n_r          (B = A S; all over sqrt(N))
+// are one real product on the matrix cores (v_mfma_f32_16x16x4_f32, float32 operands and
+// accumulation): output rows Re row_0..7, Im row_0..7; the contraction runs over chunks of four
+// inputs (antennas' noise, then users' delay-line outputs), two MFMAs per chunk and column tile
+// (real, then imaginary parts of the inputs against [Re a; Im a] and [-Im a; Re a]).  A column is
+// a sample pair, so the lane that supplies operand (k, column) draws exactly that antenna's two
+// noise samples from one Philox call (stream HPX_STREAM_NOISE + r, the samples massive_rx_kernel
+// adds) and feeds the even sample to one accumulator tile and the odd one to another: no operand
+// ever moves between lanes.  The rows and the ideal-CSI row (tap means at the path delays) are
+// then transformed kLinkRows at a time in LDS; each thread keeps the active subcarriers
+// tid + b * blockDim (b < kLinkBins) of every row in registers and detects them after the last
+// round.
 __global__ void __launch_bounds__(kLinkThreads) massive_link_kernel(MassiveArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   float2* rows = reinterpret_cast<float2*>(lds + a.k3_rows);  // [kLinkRows][rstride]
   float2* ptab = reinterpret_cast<float2*>(lds + a.k3_tab);   // FFT pass table
-  float2* bm = reinterpret_cast<float2*>(lds + a.k3_b);       // [nt][nt] B / sqrt(N)
-  float2* am = reinterpret_cast<float2*>(lds + a.k3_a);       // [nr][kMaxUsers] sigma A^T / sqrt(N)
+  float2* atab = reinterpret_cast<float2*>(lds + a.k3_atab);  // [chunk][lane] MFMA A operands
   float2* vm = reinterpret_cast<float2*>(lds + a.k3_v);       // [nt][nt] V
   float* lam = reinterpret_cast<float*>(lds + a.k3_l);
   DemapBounds* dmb = reinterpret_cast<DemapBounds*>(lds + a.k3_dmb);
@@ -468,20 +522,33 @@ __global__ void __launch_bounds__(kLinkThreads) massive_link_kernel(MassiveArgs 
   const int64_t ld = blk / a.words;
   const uint32_t drop = static_cast<uint32_t>(a.first_drop + ld);
   const int n = a.n, nt = a.nt, nr = a.nr, rs = a.k3_rstride, half = a.n >> 1;
+  const int noise_chunks = (nr + 3) >> 2, chunks = noise_chunks + ((nt + 3) >> 2);
   const double np = a.noise_power[ld];
   const float rx_scale = static_cast<float>(1.0 / sqrt(static_cast<double>(n)));
   const float sigma_scale = static_cast<float>(sqrt(0.5 * np) / sqrt(static_cast<double>(n)));
   const float s2 = static_cast<float>(np);
   fill_fft_pass_table(ptab, a.log2n);
+  // A operand of chunk c at lane l: row i = l % 16, input k = l / 16; entry (a1, a2) for the real
+  // and the imaginary parts of the inputs, a = the row's user's coefficient of input 4 c + k.
+  // Output row i = 4 g + j (D register j of lanes 16 g ..) carries slot 4 j + g = 2 user + comp,
+  // so that register j holds users 2 j and 2 j + 1: the transform rounds (users 0-2, 3-5, 6-7)
+  // release registers 0, then 1 and 2, then 3.
   const float2* da = a.det_a + ld * a.da_stride;
-  for (int e = tid; e < nr * kMaxUsers; e += nth) {
-    const int r = e / kMaxUsers, i = e - r * kMaxUsers;
-    am[e] = i < nt ? cscale(da[i * nr + r], sigma_scale) : make_float2(0.f, 0.f);
+  for (int e = tid; e < chunks * 64; e += nth) {
+    const int c = e >> 6, l = e & 63;
+    const int i = l & 15, k = l >> 4;
+    const int slot = 4 * (i & 3) + (i >> 2), user = slot >> 1, comp = slot & 1;
+    float2 co = make_float2(0.f, 0.f);
+    if (c < noise_chunks) {
+      const int r = 4 * c + k;
+      if (r < nr && user < nt) co = cscale(da[user * nr + r], sigma_scale);
+    } else {
+      const int t = 4 * (c - noise_chunks) + k;
+      if (t < nt && user < nt) co = cscale(da[nt * nr + user * nt + t], rx_scale);
+    }
+    atab[e] = comp == 0 ? make_float2(co.x, -co.y) : make_float2(co.y, co.x);
   }
-  for (int e = tid; e < nt * nt; e += nth) {
-    bm[e] = cscale(da[nt * nr + e], rx_scale);
-    vm[e] = a.det_v[ld * nt * nt + e];
-  }
+  for (int e = tid; e < nt * nt; e += nth) vm[e] = a.det_v[ld * nt * nt + e];
   for (int i = tid; i < nt; i += nth) lam[i] = a.det_l[ld * nt + i];
   if (tid == 0) {
 #pragma unroll
@@ -491,59 +558,53 @@ __global__ void __launch_bounds__(kLinkThreads) massive_link_kernel(MassiveArgs 
     }
   }
   __syncthreads();
-  // user rows at samples 2 j, 2 j + 1 of the data window, j = tid + p * blockDim
+  // column tile q of wave v: sample pairs P = (q * waves + v) * 16 + column, i.e. samples 2 P
+  // (accumulator 0) and 2 P + 1 (accumulator 1) of the data window; D register j of lane l is
+  // output row 4 (l / 16) + j at column l % 16: component (l / 16) % 2 of user (4 j + l / 16) / 2
+  const int lane = tid & 63, wave = tid >> 6, waves = nth >> 6;
+  const int kq = lane >> 4, col = lane & 15;
   const int64_t win0 = static_cast<int64_t>(w) * (n + a.cp) + a.cp;
+  const bool aligned = (win0 & 1) == 0;
   const float2* zd = a.z + (ld * nt * a.words + w) * static_cast<int64_t>(n);
-  float2 acc[kMaxUsers][2 * kLinkPairs];
+  floatx4 acc[kLinkPasses][2];
 #pragma unroll
-  for (int i = 0; i < kMaxUsers; ++i)
-#pragma unroll
-    for (int q = 0; q < 2 * kLinkPairs; ++q) acc[i][q] = make_float2(0.f, 0.f);
-#pragma unroll
-  for (int p = 0; p < kLinkPairs; ++p) {
-    const int j = tid + p * nth;
-    if (j >= half) continue;
-    for (int t = 0; t < nt; ++t) {
-      const float4 zz = *reinterpret_cast<const float4*>(zd + static_cast<int64_t>(t) * a.words * n + 2 * j);
-#pragma unroll
-      for (int i = 0; i < kMaxUsers; ++i) {
-        if (i < nt) {
-          const float2 b = bm[i * nt + t];
-          acc[i][2 * p] = cfma(b, make_float2(zz.x, zz.y), acc[i][2 * p]);
-          acc[i][2 * p + 1] = cfma(b, make_float2(zz.z, zz.w), acc[i][2 * p + 1]);
+  for (int q = 0; q < kLinkPasses; ++q) {
+    acc[q][0] = floatx4{0.f, 0.f, 0.f, 0.f};
+    acc[q][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int pbase = (q * waves + wave) * 16;
+    if (pbase >= half) continue;  // whole tiles: N / 2 is a multiple of 16
+    const int pair = pbase + col;
+    for (int c = 0; c < noise_chunks; ++c) {
+      const int r = 4 * c + kq;
+      float2 n0 = make_float2(0.f, 0.f), n1 = make_float2(0.f, 0.f);
+      if (r < nr) {
+        // noise sample m of antenna r: words (0, 1) or (2, 3) of Philox call m / 2
+        const int64_t m = win0 + 2 * pair;
+        const uint4 wa = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>(m >> 1));
+        if (aligned) {
+          n0 = complex_normal_f32(wa.x, wa.y);
+          n1 = complex_normal_f32(wa.z, wa.w);
+        } else {
+          const uint4 wb = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>((m + 1) >> 1));
+          n0 = complex_normal_f32(wa.z, wa.w);
+          n1 = complex_normal_f32(wb.x, wb.y);
         }
       }
+      const float2 av = atab[c * 64 + lane];
+      acc[q][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, n0.x, acc[q][0], 0, 0, 0);
+      acc[q][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, n1.x, acc[q][1], 0, 0, 0);
+      acc[q][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, n0.y, acc[q][0], 0, 0, 0);
+      acc[q][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, n1.y, acc[q][1], 0, 0, 0);
     }
-  }
-  // noise sample m of antenna r: words (0, 1) or (2, 3) of Philox call m / 2 of stream
-  // HPX_STREAM_NOISE + r; an even window start makes each pair one call
-  const bool aligned = (win0 & 1) == 0;
-  for (int r = 0; r < nr; ++r) {
-    float2 nz[2 * kLinkPairs];
-#pragma unroll
-    for (int p = 0; p < kLinkPairs; ++p) {
-      const int j = tid + p * nth;
-      nz[2 * p] = nz[2 * p + 1] = make_float2(0.f, 0.f);
-      if (j >= half) continue;
-      const int64_t m = win0 + 2 * j;
-      const uint4 wa = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>(m >> 1));
-      if (aligned) {
-        nz[2 * p] = complex_normal_f32(wa.x, wa.y);
-        nz[2 * p + 1] = complex_normal_f32(wa.z, wa.w);
-      } else {
-        const uint4 wb = stream_call(a.seed, drop, HPX_STREAM_NOISE + r, static_cast<uint32_t>((m + 1) >> 1));
-        nz[2 * p] = complex_normal_f32(wa.z, wa.w);
-        nz[2 * p + 1] = complex_normal_f32(wb.x, wb.y);
-      }
-    }
-    const float2* ar = am + r * kMaxUsers;
-#pragma unroll
-    for (int i = 0; i < kMaxUsers; ++i) {
-      if (i < nt) {
-        const float2 c = ar[i];
-#pragma unroll
-        for (int q = 0; q < 2 * kLinkPairs; ++q) acc[i][q] = cfma(c, nz[q], acc[i][q]);
-      }
+    for (int c = noise_chunks; c < chunks; ++c) {
+      const int t = 4 * (c - noise_chunks) + kq;
+      float4 zz = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (t < nt) zz = *reinterpret_cast<const float4*>(zd + static_cast<int64_t>(t) * a.words * n + 2 * pair);
+      const float2 av = atab[c * 64 + lane];
+      acc[q][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, zz.x, acc[q][0], 0, 0, 0);
+      acc[q][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, zz.z, acc[q][1], 0, 0, 0);
+      acc[q][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, zz.y, acc[q][0], 0, 0, 0);
+      acc[q][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, zz.w, acc[q][1], 0, 0, 0);
     }
   }
   // transforms: rows v0 .. v0 + kLinkRows - 1 of (users 0 .. nt - 1, the CSI row) per round
@@ -561,16 +622,21 @@ __global__ void __launch_bounds__(kLinkThreads) massive_link_kernel(MassiveArgs 
     if (v0 > nt) return;
     const int cnt = min(kLinkRows, nt + 1 - v0);
     const bool csi_here = nt < v0 + kLinkRows;
+    // lane (kq, col), register j: component kq % 2 of user (4 j + kq) / 2
+    {
+      float* rowf = reinterpret_cast<float*>(rows) + (kq & 1);
 #pragma unroll
-    for (int i = v0; i < v0 + kLinkRows && i < kMaxUsers; ++i) {
-      if (i < nt) {
-        float2* row = rows + (i - v0) * rs;
+      for (int j = v0 / 2; j <= (v0 + kLinkRows - 1) / 2 && j < 4; ++j) {
+        const int user = (4 * j + kq) >> 1;
+        if (user < v0 || user >= v0 + kLinkRows || user >= nt) continue;
+        float* rb = rowf + 2 * (user - v0) * rs;
 #pragma unroll
-        for (int p = 0; p < kLinkPairs; ++p) {
-          const int j = tid + p * nth;
-          if (j < half)
-            *reinterpret_cast<float4*>(row + 2 * j) =
-                make_float4(acc[i][2 * p].x, acc[i][2 * p].y, acc[i][2 * p + 1].x, acc[i][2 * p + 1].y);
+        for (int q = 0; q < kLinkPasses; ++q) {
+          const int pbase = (q * waves + wave) * 16;
+          if (pbase >= half) continue;
+          const int sm = 2 * (pbase + col);
+          rb[2 * sm] = acc[q][0][j];
+          rb[2 * sm + 2] = acc[q][1][j];
         }
       }
     }
@@ -715,7 +781,7 @@ int make_massive(const hpx_massive_link* l, MassiveArgs* out) {
   HPX_REQUIRE(a.dx <= a.n, "hpx_massive_run: delay spread too long");
   a.cpl = std::min(a.cp, a.maxd);
   a.x0 = a.dx + a.cpl;
-  a.xs = a.x0 + a.n;
+  a.xs = a.x0 + a.n + (a.n >> kFftPadShift);  // the word in the FFT's padded extent
   a.keep_below = static_cast<int>(std::min<int64_t>(std::min<int64_t>(1 + l->max_delay, a.frame_len), a.n));
   a.bits_words = (a.active * a.bps + 63) >> 5;
   // K0 LDS
@@ -736,13 +802,13 @@ int make_massive(const hpx_massive_link* l, MassiveArgs* out) {
   // K1 LDS
   o = 0;
   a.k1_x = 0;
-  o += align16(static_cast<int64_t>(a.xs) * 8);
-  a.k1_tw = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(a.n) * 8);
+  o += align16(static_cast<int64_t>(kTxUsers) * a.xs * 8);
+  a.k1_tab = static_cast<int>(o);
+  o += align16(static_cast<int64_t>(fft_pass_table_size(a.n)) * 8);
   a.k1_coef = static_cast<int>(o);
   o += align16(static_cast<int64_t>(a.paths) * a.nb * kTaylorCoeffs * 8);
   a.k1_bits = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(a.bits_words) * 4);
+  o += align16(static_cast<int64_t>(kTxUsers) * a.bits_words * 4);
   a.k1_cons = static_cast<int>(o);
   o += align16(static_cast<int64_t>(a.order) * 8);
   a.k1_dly = static_cast<int>(o);
@@ -753,7 +819,7 @@ int make_massive(const hpx_massive_link* l, MassiveArgs* out) {
   a.k2_tw = static_cast<int>(align16(static_cast<int64_t>(kRxRows + 1) * a.n * 8));
   a.k2_lds = static_cast<int>(a.k2_tw + align16(static_cast<int64_t>(a.n) * 8));
   // K3 LDS: the transform rows (the padded extent N + N / 32 each; later the word's bits), the
-  // pass table, B, sigma A^T, V, lambda, the demapper's boundaries, the error partials
+  // pass table, the MFMA A operands, V, lambda, the demapper's boundaries, the error partials
   a.da_stride = a.nt * a.nr + a.nt * a.nt;
   a.dbounds = demap_bounds(a.mod);
   a.k3_rstride = a.n + (a.n >> kFftPadShift);
@@ -763,10 +829,8 @@ int make_massive(const hpx_massive_link* l, MassiveArgs* out) {
                                  static_cast<int64_t>(a.nt) * a.bits_words * 4));
   a.k3_tab = static_cast<int>(o);
   o += align16(static_cast<int64_t>(fft_pass_table_size(a.n)) * 8);
-  a.k3_b = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(a.nt) * a.nt * 8);
-  a.k3_a = static_cast<int>(o);
-  o += align16(static_cast<int64_t>(a.nr) * kMaxUsers * 8);
+  a.k3_atab = static_cast<int>(o);
+  o += align16(static_cast<int64_t>((a.nr + 3) / 4 + (a.nt + 3) / 4) * 64 * 8);
   a.k3_v = static_cast<int>(o);
   o += align16(static_cast<int64_t>(a.nt) * a.nt * 8);
   a.k3_l = static_cast<int>(o);
@@ -780,7 +844,7 @@ int make_massive(const hpx_massive_link* l, MassiveArgs* out) {
     set_error("hpx_massive_run: the link needs more than 160 KB of LDS per workgroup");
     return HPX_ERR_UNSUPPORTED;
   }
-  if (a.n > kFftMaxPerThread * kRxThreads || a.n > 2 * kLinkPairs * kLinkThreads ||
+  if (a.n > kFftMaxPerThread * kRxThreads || a.n > 2 * 16 * kLinkPasses * (kLinkThreads / 64) || a.n > kTxSamples * kTxThreads ||
       a.active > kLinkBins * kLinkThreads) {
     set_error("hpx_massive_run: %d subcarriers exceed one workgroup's FFT capacity", a.n);
     return HPX_ERR_UNSUPPORTED;
@@ -855,7 +919,8 @@ extern "C" int hpx_massive_run(const hpx_massive_link* link_host, const int32_t*
   }
   const unsigned nd = static_cast<unsigned>(num_drops);
   hipLaunchKernelGGL(massive_prologue_kernel, dim3(nd), dim3(256), a.k0_lds, s, a);
-  hipLaunchKernelGGL(massive_tx_kernel, dim3(nd * a.nt * a.words), dim3(kTxThreads), a.k1_lds, s, a);
+  const unsigned tx_groups = static_cast<unsigned>((a.nt + kTxUsers - 1) / kTxUsers);
+  hipLaunchKernelGGL(massive_tx_kernel, dim3(nd * tx_groups), dim3(kTxThreads), a.k1_lds, s, a);
   if (a.grid) {  // the per-antenna grid and the CSI factor, on request
     const int64_t groups = (a.nr + kRxGroup - 1) / kRxGroup;
     a.num_pairs = static_cast<int64_t>(nd) * a.words;
