@@ -101,8 +101,10 @@ __device__ __forceinline__ float2 complex_normal_f32(uint32_t a, uint32_t b) {
   return make_float2(rho * c, rho * s);
 }
 
-// Standard normal CDF (the uniform transform of ConsistentUniform).
-__device__ __forceinline__ double normal_cdf(double z) { return 0.5 * erfc(-z * 0.70710678118654752440); }
+// Standard normal CDF (the uniform transform of ConsistentUniform).  Out of line: inlined into a
+// kernel's prologue loop, erfc's float64 polynomial constants are hoisted into ~60 registers for
+// the whole loop, which pushes a 128-register kernel into spilling.
+__device__ __noinline__ double normal_cdf(double z) { return 0.5 * erfc(-z * 0.70710678118654752440); }
 
 }  // namespace hpx
 
