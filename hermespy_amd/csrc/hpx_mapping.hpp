@@ -195,6 +195,49 @@ __host__ __device__ inline uint32_t pam_decide_f32(int pam, float x, const Demap
   return b;
 }
 
+// pam_decide_f32 for a compile-time PAM order with the boundaries of T = 2, 4, .., PAM - 2 in
+// registers (ta[k] = a[k], tb[k] = b[k]): every comparison is evaluated (bitwise | and &, no
+// short-circuit branches), so a wavefront stays on one path; the same decisions.
+template <int PAM>
+__host__ __device__ __forceinline__ uint32_t pam_decide_f32_reg(float x, const float* ta, const float* tb) {
+#define HPX_GT(T) (x > ta[(T) / 2 - 1])
+#define HPX_LT(T) (x < tb[(T) / 2 - 1])
+#define HPX_LTN(T) (x < -ta[(T) / 2 - 1])
+#define HPX_GTN(T) (x > -tb[(T) / 2 - 1])
+  uint32_t b = x < 0.0f ? 1u : 0u;
+  if constexpr (PAM == 4) {
+    b |= (HPX_LTN(2) | HPX_GT(2)) ? 2u : 0u;
+  } else if constexpr (PAM == 8) {
+    b |= (HPX_LTN(4) | HPX_GT(4)) ? 2u : 0u;
+    b |= (HPX_LTN(6) | (HPX_GTN(2) & HPX_LT(2)) | HPX_GT(6)) ? 4u : 0u;
+  } else if constexpr (PAM == 16) {
+    b |= (HPX_LTN(8) | HPX_GT(8)) ? 2u : 0u;
+    b |= (HPX_LTN(12) | (HPX_GTN(4) & HPX_LT(4)) | HPX_GT(12)) ? 4u : 0u;
+    b |= (HPX_LTN(14) | (HPX_GTN(10) & HPX_LTN(6)) | (HPX_GTN(2) & HPX_LT(2)) | (HPX_GT(6) & HPX_LT(10)) |
+          HPX_GT(14))
+             ? 8u
+             : 0u;
+  }
+#undef HPX_GT
+#undef HPX_LT
+#undef HPX_LTN
+#undef HPX_GTN
+  return __builtin_isfinite(x) ? b : 0u;
+}
+
+// demap_symbol_f32 of a complex constellation of compile-time PAM order (2, 4, 8, 16: QPSK to
+// 256-QAM), boundaries in registers as for pam_decide_f32_reg.
+template <int PAM>
+__host__ __device__ __forceinline__ uint32_t demap_symbol_f32_reg(float re, float im, const float* ta, const float* tb) {
+  constexpr int nb = PAM == 2 ? 1 : PAM == 4 ? 2 : PAM == 8 ? 3 : 4;  // bits per axis
+  const uint32_t br = pam_decide_f32_reg<PAM>(re, ta, tb);
+  const uint32_t bi = pam_decide_f32_reg<PAM>(im, ta, tb);
+  uint32_t out = 0;
+#pragma unroll
+  for (int i = 0; i < nb; ++i) out |= (((br >> i) & 1u) << (2 * i)) | (((bi >> i) & 1u) << (2 * i + 1));
+  return out;
+}
+
 // demap_symbol of a complex constellation for float32 components (bit-identical decisions).
 __host__ __device__ inline uint32_t demap_symbol_f32(const Modulation& m, const DemapBounds* d, float re, float im) {
   const uint32_t br = pam_decide_f32(m.pam, re, d);

@@ -80,7 +80,7 @@ struct MassiveArgs {
   uint64_t seed;
   int64_t first_drop;
   float2* spatial;  // [drop][nr][nt]
-  float2* coef;     // [drop][L][nb][5]
+  float2* coef;     // [drop][L][nb][kTaylorCoeffs]
   float2* abar;     // [drop][L][words]
   float2* det_v;    // [drop][nt][nt]
   float2* det_a;    // [drop][nt][nr] A, then [nt][nt] B = A S
@@ -688,34 +688,50 @@ __global__ void __launch_bounds__(kLinkThreads) massive_link_kernel(MassiveArgs 
     }
   }
   __syncthreads();
-  // per subcarrier: x = V diag(conj(c) / (|c|^2 lambda + s2)) u, decisions, bit errors
+  // per subcarrier: x = V diag(conj(c) / (|c|^2 lambda + s2)) u, decisions, bit errors; one loop
+  // per PAM order, with the demapper's boundaries in registers
   int err = 0;
+  auto detect = [&](auto pam_c) {
+    constexpr int PAM = decltype(pam_c)::value;
+    float ta[PAM / 2], tbd[PAM / 2];
 #pragma unroll
-  for (int b = 0; b < kLinkBins; ++b) {
-    const int k = tid + b * nth;
-    if (k >= a.active) continue;
-    const float c2 = cabs2(c[b]);
-    float2 g[kMaxUsers];
+    for (int k = 0; k < PAM / 2; ++k) {
+      ta[k] = dmb->a[k];
+      tbd[k] = dmb->b[k];
+    }
 #pragma unroll
-    for (int i = 0; i < kMaxUsers; ++i) {
-      g[i] = make_float2(0.f, 0.f);
-      if (i < nt) {
-        const float den = fmaf(c2, lam[i], s2);
-        const float gi = (lam[i] > 0.0f || s2 > 0.0f) && den > 0.0f ? 1.0f / den : 0.0f;
-        g[i] = cscale(cmulc(c[b], u[i][b]), gi);
+    for (int b = 0; b < kLinkBins; ++b) {
+      const int k = tid + b * nth;
+      if (k >= a.active) continue;
+      const float c2 = cabs2(c[b]);
+      float2 g[kMaxUsers];
+#pragma unroll
+      for (int i = 0; i < kMaxUsers; ++i) {
+        g[i] = make_float2(0.f, 0.f);
+        if (i < nt) {
+          const float den = fmaf(c2, lam[i], s2);
+          const float gi = (((lam[i] > 0.0f) | (s2 > 0.0f)) & (den > 0.0f)) ? 1.0f / den : 0.0f;
+          g[i] = cscale(cmulc(c[b], u[i][b]), gi);
+        }
+      }
+      const int sbit = bo + k * a.bps;
+      for (int t = 0; t < nt; ++t) {
+        float2 xh = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int i = 0; i < kMaxUsers; ++i)
+          if (i < nt) xh = cfma(vm[t * nt + i], g[i], xh);
+        const uint32_t rx = demap_symbol_f32_reg<PAM>(xh.x, xh.y, ta, tbd);
+        const uint32_t tx = symbol_bits_at(bits + t * bw, sbit, a.bps);
+        err += __popc(rx ^ tx);
+        if (a.equalized) a.equalized[((ld * nt + t) * a.words + w) * static_cast<int64_t>(a.active) + k] = xh;
       }
     }
-    const int sbit = bo + k * a.bps;
-    for (int t = 0; t < nt; ++t) {
-      float2 xh = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int i = 0; i < kMaxUsers; ++i)
-        if (i < nt) xh = cfma(vm[t * nt + i], g[i], xh);
-      const uint32_t rx = demap_symbol_f32(a.mod, dmb, xh.x, xh.y);
-      const uint32_t tx = symbol_bits_at(bits + t * bw, sbit, a.bps);
-      err += __popc(rx ^ tx);
-      if (a.equalized) a.equalized[((ld * nt + t) * a.words + w) * static_cast<int64_t>(a.active) + k] = xh;
-    }
+  };
+  switch (a.mod.pam) {
+    case 2: detect(std::integral_constant<int, 2>{}); break;
+    case 4: detect(std::integral_constant<int, 4>{}); break;
+    case 8: detect(std::integral_constant<int, 8>{}); break;
+    default: detect(std::integral_constant<int, 16>{}); break;
   }
   for (int off = 32; off > 0; off >>= 1) err += __shfl_down(err, off, 64);
   if ((tid & 63) == 0) red[tid >> 6] = err;

@@ -42,7 +42,7 @@ int check_launch(const char* what) {
 }
 
 // Largest power-of-two Taylor block (log2) keeping the truncation bound
-// sqrt(terms) x^(P+1) / (P+1)! <= 1e-8 with x = max_rate * B / 2; 0 = direct.
+// sqrt(terms) x^(P+1) / (P+1)! <= 1e-6 with x = max_rate * B / 2; 0 = direct.
 int taylor_log2_block(double max_rate, int terms, int max_log2) {
   double fact = 1.0;
   for (int k = 2; k <= kTaylorCoeffs; ++k) fact *= k;
@@ -50,7 +50,7 @@ int taylor_log2_block(double max_rate, int terms, int max_log2) {
   int best = 0;
   for (int lb = 1; lb <= max_log2; ++lb) {
     const double x = fabs(max_rate) * 0.5 * static_cast<double>(1 << lb);
-    if (amp * pow(x, kTaylorCoeffs) / fact <= 1e-8) best = lb;
+    if (amp * pow(x, kTaylorCoeffs) / fact <= 1e-6) best = lb;
   }
   return best;
 }
@@ -340,7 +340,20 @@ extern "C" int hpx_demap_hard_f32_host(const hpx_c64* symbols_host, int64_t num_
   }
   const DemapBounds d = demap_bounds(m);
   for (int64_t i = 0; i < num_symbols; ++i) {
-    const uint32_t b = demap_symbol_f32(m, &d, symbols_host[i].re, symbols_host[i].im);
+    const float re = symbols_host[i].re, im = symbols_host[i].im;
+    const uint32_t b = demap_symbol_f32(m, &d, re, im);
+    // the kernels' compile-time-order form must decide the same
+    uint32_t bt;
+    switch (m.pam) {
+      case 2: bt = demap_symbol_f32_reg<2>(re, im, d.a, d.b); break;
+      case 4: bt = demap_symbol_f32_reg<4>(re, im, d.a, d.b); break;
+      case 8: bt = demap_symbol_f32_reg<8>(re, im, d.a, d.b); break;
+      default: bt = demap_symbol_f32_reg<16>(re, im, d.a, d.b); break;
+    }
+    if (bt != b) {
+      set_error("hpx_demap_hard_f32_host: demapper forms disagree at symbol %lld", static_cast<long long>(i));
+      return HPX_ERR_INVALID;
+    }
     for (int k = 0; k < m.bps; ++k) bits_host[i * m.bps + k] = static_cast<uint8_t>((b >> k) & 1u);
   }
   return HPX_OK;

@@ -134,7 +134,9 @@ int hpx_demap_hard_host(const hpx_c128* symbols_host, int64_t num_symbols, int m
 /* Host twin of the float32 decision path the detection kernels use for complex constellations
  * (4, 16, 64, 256): every threshold of the reference's piecewise LLRs replaced by the float32
  * boundary of the float64 amplitude it is compared in (psk_qam_mapping.py:301-497); the same
- * decisions as hpx_demap_hard_host on the float64 values of the inputs. */
+ * decisions as hpx_demap_hard_host on the float64 values of the inputs.  Both forms of the
+ * decision the kernels compile (run-time and compile-time constellation order) are evaluated;
+ * HPX_ERR_INVALID if they ever disagree. */
 int hpx_demap_hard_f32_host(const hpx_c64* symbols_host, int64_t num_symbols, int modulation_order, uint8_t* bits_host);
 
 /* OFDM modulation of a [num_streams][words][num_subcarriers] grid into

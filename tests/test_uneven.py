@@ -84,7 +84,7 @@ def test_lean_drops_need_a_workspace(native_lib):
     assert native_lib.hpx_run_batch_lds_bytes(ctypes.byref(big)) == 0
     ws = native_lib.hpx_uneven_workspace_elements(ctypes.byref(big))
     tables_len = 2048 + 2047
-    assert ws > tables_len and (ws - tables_len) % (23 * 5) == 0
+    assert ws > tables_len and (ws - tables_len) % (23 * 4) == 0  # 4 coefficients of the cubic
     fewer = _c_link(DropShape(8, 2048, 5, 2.0, 100.0, 16, 10.0).link())
     assert (native_lib.hpx_uneven_workspace_elements(ctypes.byref(fewer)) - tables_len) * 23 == (ws - tables_len) * 5
     for shape in ((8, 1024), (4, 2048), (1, 128)):
