@@ -31,18 +31,9 @@ SEED = 20261202
 
 
 def algorithmic_flops_per_drop(cfg) -> float:
-    """Per-drop work of the reference algorithm: 8 flops per complex MAC of the
-    TDL convolution (L M Ntx), spatial mix (Nrx Ntx (M + D)), CSI (L words N) and
-    4x4 MMSE detection (133 CMAC per resource element), plus 5 N log2 N flops per
-    FFT of every transmit and receive stream and word."""
+    """Per-drop work of the reference algorithm (LinkConfig.algorithmic_flops)."""
 
-    m = cfg.samples_per_frame
-    tdl = cfg.num_paths * m * cfg.num_tx
-    smix = cfg.num_rx * cfg.num_tx * (m + cfg.max_delay)
-    csi = cfg.num_paths * cfg.words * cfg.num_subcarriers
-    det = 133 * cfg.words * cfg.num_subcarriers
-    fft = 5 * cfg.num_subcarriers * np.log2(cfg.num_subcarriers) * cfg.words * (cfg.num_tx + cfg.num_rx)
-    return 8.0 * (tdl + smix + csi + det) + fft
+    return cfg.algorithmic_flops
 
 
 def _oracle_drops(args):
@@ -246,9 +237,16 @@ def uneven_leg(drops: int = 8192, reps: int = 5) -> dict:
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / reps
     bits = float(ub.bits.sum())
+    flops = float(sum(c.algorithmic_flops for c in ub.links))
+    achieved = flops / (ms / 1e3) / 1e12
     return {"workload": "uneven mix (8x8 at 2048 subcarriers in lean classes)", "drops": drops,
             "lean_drops": ub.num_lean, "ms_per_launch": ms,
             "drops_per_s": drops / (ms / 1e3), "bits_per_s": bits / (ms / 1e3), "classes": ub.num_classes,
+            "launches": int(sum(len(lane) for lane in ub.lanes)), "streams": len(ub.lanes),
+            "stream_cost_ms": [1e3 * x for x in ub.stream_cost],
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "flops_per_launch": flops,
+                         "note": "the population's algorithmic flops over the graph's span"},
             "workgroup_choice": tuning["choice"]}
 
 

@@ -112,6 +112,24 @@ class LinkConfig:
     def bits_per_drop(self) -> int:
         return self.num_tx * self.num_data * self.bits_per_symbol
 
+    @property
+    def algorithmic_flops(self) -> float:
+        """Per-drop work of the reference algorithm, the roofline's numerator: 8 flops per complex
+        MAC of the tapped delay line (paths x frame samples x transmit streams), the spatial mix
+        (Nrx Ntx (frame + max delay)), the ideal-CSI DFT over the taps (paths x words x N) and the
+        per-element MMSE detection (Gram Ntx^2 Nrx, Cholesky Ntx^3 / 3, matched filter Ntx Nrx and two
+        triangular solves 2 Ntx^2 complex MACs: 133 at 4x4), plus 5 N log2 N flops per FFT of every
+        transmit and receive stream and word."""
+
+        m = self.samples_per_frame
+        nt, nr, n = self.num_tx, self.num_rx, self.num_subcarriers
+        tdl = self.num_paths * m * nt
+        smix = nr * nt * (m + self.max_delay)
+        csi = self.num_paths * self.words * n
+        det = (nt * nt * nr + nt ** 3 // 3 + nt * nr + 2 * nt * nt) * self.words * n
+        fft = 5 * n * np.log2(n) * self.words * (nt + nr)
+        return 8.0 * (tdl + smix + csi + det) + fft
+
     def draw_doppler(self, first_drop: int, count: int) -> np.ndarray | None:
         """Per-drop doppler_frequency (2 pi f_D, NLOS and LOS alike) of drops first_drop.., or None.
 

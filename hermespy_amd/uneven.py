@@ -79,6 +79,35 @@ def _c_link(cfg: LinkConfig, workgroup_size: int = 0) -> _lib.HpxLink:
                         float(cfg.device_power), int(workgroup_size))
 
 
+def schedule_launches(classes, num_classes: int, cost, num_streams: int) -> tuple:
+    """Launch pieces per stream.  A class costlier than half a stream's share of the total is cut
+    into equal drop ranges (a piece is the class's descriptor over fewer plan slots); the pieces
+    then go, longest first, to the stream with the least work so far (longest-processing-time
+    scheduling), so that the streams end together instead of one stream's last long class running
+    alone.  Returns (pieces per stream in launch order, summed cost per stream)."""
+
+    cost = np.asarray(cost, dtype=np.float64)
+    share = float(cost[:num_classes].sum()) / (2 * num_streams)
+    pieces = []
+    for k in range(num_classes):
+        c = classes[k]
+        parts = 1 if share <= 0 else int(min(c.count, max(1, np.ceil(cost[k] / share))))
+        cuts = np.rint(np.linspace(0, c.count, parts + 1)).astype(np.int64)
+        for lo, hi in zip(cuts[:-1], cuts[1:]):
+            if hi > lo:
+                pc = _lib.HpxUnevenClass(c.first + int(lo), int(hi - lo), c.num_tx, c.threads, c.pairs, c.lean,
+                                         c.lds_bytes)
+                pieces.append((cost[k] * (hi - lo) / c.count, pc))
+    pieces.sort(key=lambda t: -t[0])
+    load = np.zeros(num_streams)
+    lanes = [[] for _ in range(num_streams)]
+    for w, pc in pieces:
+        j = int(np.argmin(load))
+        load[j] += w
+        lanes[j].append(pc)
+    return lanes, load.tolist()
+
+
 class UnevenBatch:
     """Device-resident plans of one mixed population (drop k of the population is Philox drop
     first_drop + k)."""
@@ -160,11 +189,27 @@ class UnevenBatch:
         self.order = order
         self.num_classes = nc.value
         self.classes = classes
+        # class costs for the stream schedule: the population's algorithmic flops until tune()
+        # replaces them with measured launch times
+        flops = np.array([c.algorithmic_flops for c in self.links])
+        self.class_cost = np.array([flops[order[classes[k].first:classes[k].first + classes[k].count]].sum()
+                                    for k in range(self.num_classes)])
+        self.schedule()
         self.graph = None
 
+    def schedule(self) -> list:
+        """Spread the class launches over the streams (schedule_launches with the classes' costs);
+        returns the streams' summed costs."""
+
+        lanes, load = schedule_launches(self.classes, self.num_classes, self.class_cost, len(self.streams))
+        self.lanes = [(_lib.HpxUnevenClass * len(lane))(*lane) for lane in lanes if lane]
+        self.stream_cost = load
+        self.graph = None
+        return load
+
     def run(self) -> None:
-        """All classes, asynchronous on the current stream: forked over the side streams in class
-        order (heaviest first, round robin) and joined back."""
+        """All classes, asynchronous on the current stream: forked over the side streams as
+        scheduled (schedule()) and joined back."""
 
         cur = torch.cuda.current_stream(self.device)
         if len(self.streams) <= 1 or self.num_classes <= 1:
@@ -175,10 +220,8 @@ class UnevenBatch:
         fork.record(cur)
         for st in self.streams:
             st.wait_event(fork)
-        for k in range(self.num_classes):
-            st = self.streams[k % len(self.streams)]
-            _lib.check(self.lib.hpx_uneven_run(self.plans.data_ptr(), ctypes.byref(self.classes[k]), 1,
-                                               st.cuda_stream))
+        for st, lane in zip(self.streams, self.lanes):
+            _lib.check(self.lib.hpx_uneven_run(self.plans.data_ptr(), lane, len(lane), st.cuda_stream))
         for st in self.streams:
             join = torch.cuda.Event()
             join.record(st)
@@ -250,5 +293,10 @@ def tune(batch: UnevenBatch, reps: int = 3, verbose: bool = False) -> dict:
             print(f"tune {key}: " + ", ".join(f"{t}: {1e3 * v:.3f} ms" for t, v in times.items()) + f" -> {best[key]}",
                   flush=True)
     batch.prepare(best)
+    # the chosen configurations' launch times become the classes' costs in the stream schedule
+    batch.class_cost = np.array([timings[batch.class_key(int(batch.order[batch.classes[k].first]))]
+                                 [best[batch.class_key(int(batch.order[batch.classes[k].first]))]]
+                                 for k in range(batch.num_classes)])
+    batch.schedule()
     return {"choice": {f"{a}x{a}-{n}": t for (a, n), t in best.items()},
             "ms": {f"{a}x{a}-{n}": {str(t): 1e3 * v for t, v in tt.items()} for (a, n), tt in timings.items()}}

@@ -281,3 +281,37 @@ def test_every_kernel_class_matches_the_oracle(cuda):
     ora = oracle_links(links, 31, [4000 + i for i in range(len(shapes))], [s.snr_db for s in shapes])
     for i, s in enumerate(shapes):
         check_drop(eqs[i], err[i], ora[i], s.modulation_order, 1e-4, (s.antennas, s.num_subcarriers))
+
+
+def test_launch_schedule_covers_every_slot_once_and_balances():
+    """schedule_launches cuts classes costlier than half a stream's share into drop ranges and
+    assigns the pieces longest first to the least-loaded stream: every plan slot is launched
+    exactly once, with its class's descriptor, and no stream exceeds the lightest by more than the
+    largest piece (the LPT bound)."""
+
+    from hermespy_amd import _lib
+    from hermespy_amd.uneven import schedule_launches
+
+    rng = np.random.default_rng(5)
+    counts = rng.integers(1, 500, 20)
+    firsts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    classes = [_lib.HpxUnevenClass(int(f), int(c), int(rng.choice([1, 2, 4, 8])), 256, 1, 0, 4096)
+               for f, c in zip(firsts, counts)]
+    cost = counts * rng.uniform(0.5, 20.0, counts.size)
+    lanes, load = schedule_launches(classes, len(classes), cost, 4)
+    slots = np.zeros(int(counts.sum()), dtype=np.int64)
+    biggest = 0.0
+    for lane in lanes:
+        for pc in lane:
+            k = int(np.searchsorted(firsts, pc.first, side="right") - 1)
+            assert pc.first + pc.count <= firsts[k] + counts[k]
+            assert (pc.num_tx, pc.threads, pc.lds_bytes) == (classes[k].num_tx, 256, 4096)
+            slots[pc.first:pc.first + pc.count] += 1
+            biggest = max(biggest, cost[k] * pc.count / counts[k])
+    assert np.all(slots == 1)
+    assert abs(sum(load) - cost.sum()) < 1e-6 * cost.sum()
+    assert max(load) - min(load) <= biggest + 1e-9
+    assert biggest <= cost.sum() / (2 * 4) + cost.max() / counts.min()  # split pieces stay near the share
+    # one stream still launches every slot once
+    lanes1, _ = schedule_launches(classes, len(classes), cost, 1)
+    assert sum(pc.count for pc in lanes1[0]) == counts.sum()
