@@ -22,10 +22,12 @@ import numpy as np
 METRIC = "MC drops/s (4x4 1024-SC OFDM, MMSE) at BER parity; learned vs LMMSE est."
 FP32_VECTOR_PEAK_TFLOPS = 157.3  # published MI355X FP32 peak (packed FP32 math)
 FP32_UNPACKED_PEAK_TFLOPS = 78.6  # one FP32 FMA per lane and cycle: the kernels are built without packed FP32
+HBM_PEAK_GBPS = 8000.0  # published MI355X HBM3E bandwidth
 # HBM bytes of one fused 4x4 drop, from rocprofv3 FETCH_SIZE / WRITE_SIZE over 45 056-drop launches
 # (FETCH x2.0 and WRITE x1.0, the factors a 256 MiB device copy reads in the same process):
-# (3199.7 x 2 + 109351.6) KiB / 45056; almost all of it is register-spill stores
-HBM_BYTES_PER_DROP = (3199.73 * 2.0 + 109351.63) * 1024.0 / 45056.0
+# (2084 x 2 + 1408) KiB / 45056; the algorithmic part is ~12 B (noise power, error count) plus the
+# link tables every drop reads from L2
+HBM_BYTES_PER_DROP = (2084.0 * 2.0 + 1408.0) * 1024.0 / 45056.0
 DROPS_PER_STEP = 11 * 4096  # one step: the campaign's 4096 drops at each of its 11 SNR points, one launch
 SEED = 20261202
 
@@ -306,8 +308,19 @@ def massive_leg(drops: int = 2048, chunk: int = 1024) -> dict:
     e = err.cpu().numpy()
     idx = np.arange(drops) % snr.size
     ber = [float(e[idx == k].sum() / ((idx == k).sum() * cfg.bits_per_drop)) for k in range(snr.size)]
-    return {"workload": cfg.name, "drops": drops, "ms": ms, "drops_per_s": drops / (ms / 1e3),
-            "bits_per_s": drops * cfg.bits_per_drop / (ms / 1e3), "snr_db": snr.tolist(), "ber": ber}
+    rate = drops / (ms / 1e3)
+    mfma = cfg.mfma_flops_per_drop * rate / 1e12
+    hbm = cfg.hbm_bytes_per_drop * rate / 1e9
+    return {"workload": cfg.name, "drops": drops, "drops_per_launch": chunk, "ms": ms, "drops_per_s": rate,
+            "bits_per_s": drops * cfg.bits_per_drop / (ms / 1e3), "snr_db": snr.tolist(), "ber": ber,
+            "roofline": {"bound": "mfma", "achieved": mfma, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": mfma / FP32_VECTOR_PEAK_TFLOPS, "mfma_flops_per_drop": cfg.mfma_flops_per_drop,
+                         "note": "float32 MFMA of the receiver over the whole pipeline's time (a lower bound on "
+                                 "the receive kernel's own rate)",
+                         "hbm": {"achieved": hbm, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": hbm / HBM_PEAK_GBPS,
+                                 "bytes_per_drop": cfg.hbm_bytes_per_drop},
+                         "reference_equivalent": {"flops_per_drop": cfg.reference_flops_per_drop,
+                                                  "tflops": cfg.reference_flops_per_drop * rate / 1e12}}}
 
 
 def small_link_leg(drops_per_point: int = 1000, reps: int = 3) -> dict:

@@ -76,6 +76,41 @@ class MassiveConfig:
     def noise_power(self, snr_db) -> np.ndarray:
         return self.link().noise_power(snr_db)
 
+    @property
+    def mfma_flops_per_drop(self) -> float:
+        """Matrix-core work the receiver executes per drop: per word, N / 32 column tiles of 16
+        sample pairs, each over ceil(Nrx / 4) noise chunks and ceil(users / 4) user chunks of four
+        16x16x4 float32 MFMAs (2 x 16 x 16 x 4 flops each); every product is a term of the
+        matched filter A n or of the user mix A S z."""
+
+        chunks = -(-self.num_rx // 4) + -(-self.num_users // 4)
+        return float(self.words * (self.num_subcarriers // 32) * chunks * 4 * 2 * 16 * 16 * 4)
+
+    @property
+    def hbm_bytes_per_drop(self) -> float:
+        """Compulsory HBM traffic of a drop: the users' delay-line outputs (complex64, users x words
+        x N) written by the transmitter and read once by the receiver, the per-drop tables and the
+        error count."""
+
+        z = self.num_users * self.words * self.num_subcarriers * 8
+        return float(2 * z)
+
+    @property
+    def reference_flops_per_drop(self) -> float:
+        """Work of the reference's form of the link (per-antenna receive FFTs, the per-element
+        Gram, Cholesky, matched filter and triangular solves, the full-frame delay line and spatial
+        mix): the figure a drop would cost without the per-drop eigen form and the FFT's linearity,
+        8 flops per complex MAC and 5 N log2 N per FFT."""
+
+        lk = self.link()
+        m, nt, nr, n = lk.samples_per_frame, self.num_users, self.num_rx, self.num_subcarriers
+        tdl = lk.num_paths * m * nt
+        smix = nr * nt * (m + lk.max_delay)
+        csi = lk.num_paths * self.words * n
+        det = (nt * nt * nr + nt ** 3 // 3 + nt * nr + 2 * nt * nt) * self.words * self.active
+        fft = 5 * n * np.log2(n) * self.words * (nt + nr)
+        return 8.0 * (tdl + smix + csi + det) + fft
+
     def c_link(self) -> _lib.HpxMassiveLink:
         lk = self.link()
         return _lib.HpxMassiveLink(self.num_rx, self.num_users, self.num_subcarriers, self.active, self.words,
