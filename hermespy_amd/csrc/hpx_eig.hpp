@@ -16,6 +16,68 @@ namespace hpx {
 
 constexpr int kEigMax = 8;
 
+// Parallel-order cyclic Jacobi: a sweep is n' - 1 rounds (n' = n rounded up to even) of n' / 2
+// disjoint pivot pairs (the circle method of round-robin tournaments: index 0 fixed, the others
+// rotating one place per round; a pair with the padding index n is skipped).  The rotations of a
+// round touch disjoint rows and columns, so they commute and their parameters all come from the
+// matrix before the round: applied one after another (host) or side by side (one wavefront) they
+// give the same result.
+__host__ __device__ inline void jacobi_round_pair(int n, int round, int i, int* p, int* q) {
+  const int m = n + (n & 1);  // players 0 .. m-1; player n (odd n) sits out
+  auto player = [&](int slot) { return slot == 0 ? 0 : 1 + (slot - 1 + round) % (m - 1); };
+  int a = player(i), b = player(m - 1 - i);
+  if (a > b) {
+    const int t = a;
+    a = b;
+    b = t;
+  }
+  *p = a;
+  *q = b;
+}
+
+// The rotation that zeroes a_pq of a Hermitian A: with d = a_qq - a_pp, x = a_pq,
+// h = sqrt(d^2 + 4 |x|^2), sigma = 2 sgn(d) / (|d| + h) (sgn(0) = 1), c = 1 / sqrt(1 + sigma^2 |x|^2)
+// and z = c sigma x, the unitary J = [[c, z], [-conj(z), c]] on columns p, q makes (J^H A J)_pq = 0
+// (t = z / (c x / |x|) = sigma |x| is the smaller root of |x| t^2 + d t - |x| = 0).  No division
+// by |x| and two square roots in the dependent chain.
+struct JacobiRotation {
+  double c, zr, zi;
+  bool skip;
+};
+
+__host__ __device__ inline JacobiRotation jacobi_rotation(double app, double aqq, double xr, double xi) {
+  JacobiRotation j;
+  const double r2 = xr * xr + xi * xi;
+  j.skip = r2 == 0.0;
+  const double d = aqq - app;
+  const double h = sqrt(d * d + 4.0 * r2);
+  const double sigma = (d >= 0.0 ? 2.0 : -2.0) / (fabs(d) + h);
+  j.c = 1.0 / sqrt(1.0 + sigma * sigma * r2);
+  const double cs = j.c * sigma;
+  j.zr = cs * xr;
+  j.zi = cs * xi;
+  return j;
+}
+
+// column update of row k of a matrix (and of V = V J): m_kp' = c m_kp - conj(z) m_kq,
+// m_kq' = z m_kp + c m_kq
+__host__ __device__ inline void jacobi_cols(const JacobiRotation& j, double* mre, double* mim, int kp, int kq) {
+  const double pr = mre[kp], pi = mim[kp], qr = mre[kq], qi = mim[kq];
+  mre[kp] = j.c * pr - (j.zr * qr + j.zi * qi);
+  mim[kp] = j.c * pi - (j.zr * qi - j.zi * qr);
+  mre[kq] = (j.zr * pr - j.zi * pi) + j.c * qr;
+  mim[kq] = (j.zr * pi + j.zi * pr) + j.c * qi;
+}
+
+// row update of column k (J^H A): a_pk' = c a_pk - z a_qk, a_qk' = conj(z) a_pk + c a_qk
+__host__ __device__ inline void jacobi_rows(const JacobiRotation& j, double* are, double* aim, int pk, int qk) {
+  const double pr = are[pk], pi = aim[pk], qr = are[qk], qi = aim[qk];
+  are[pk] = j.c * pr - (j.zr * qr - j.zi * qi);
+  aim[pk] = j.c * pi - (j.zr * qi + j.zi * qr);
+  are[qk] = (j.zr * pr + j.zi * pi) + j.c * qr;
+  aim[qk] = (j.zr * pi - j.zi * pr) + j.c * qi;
+}
+
 // a: row-major n x n Hermitian (re, im interleaved, only read/updated as a
 // full matrix); on return the diagonal holds the eigenvalues.  v: n x n
 // unitary, columns are eigenvectors.  Returns the number of sweeps used.
@@ -28,6 +90,7 @@ __host__ __device__ inline int hermitian_jacobi(int n, double* are, double* aim,
       vre[i * n + j] = (i == j) ? 1.0 : 0.0;
       vim[i * n + j] = 0.0;
     }
+  const int m = n + (n & 1);
   int sweep = 0;
 #pragma unroll 1
   for (; sweep < max_sweeps; ++sweep) {
@@ -40,72 +103,59 @@ __host__ __device__ inline int hermitian_jacobi(int n, double* are, double* aim,
     }
     if (off <= 1e-30 * diag || off == 0.0) break;
 #pragma unroll 1
-    for (int p = 0; p < n - 1; ++p)
+    for (int round = 0; round < m - 1; ++round) {
+      JacobiRotation rot[kEigMax / 2];
+      int ps[kEigMax / 2], qs[kEigMax / 2];
 #pragma unroll 1
-      for (int q = p + 1; q < n; ++q) {
-        const double xr = are[p * n + q], xi = aim[p * n + q];
-        const double r = sqrt(xr * xr + xi * xi);
-        if (r == 0.0) continue;
-        // e^{i phi} = a_pq / |a_pq|
-        const double er = xr / r, ei = xi / r;
-        const double tau = (are[q * n + q] - are[p * n + p]) / (2.0 * r);
-        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-        // G = diag(1, e^{-i phi}) [[c, s], [-s, c]] on columns p, q:
-        //   col_p' = c col_p - s e^{-i phi} col_q,  col_q' = s col_p + c e^{-i phi} col_q
+      for (int i = 0; i < m / 2; ++i) {
+        jacobi_round_pair(n, round, i, &ps[i], &qs[i]);
+        const int p = ps[i], q = qs[i];
+        rot[i].skip = true;
+        if (q < n) rot[i] = jacobi_rotation(are[p * n + p], are[q * n + q], are[p * n + q], aim[p * n + q]);
+      }
 #pragma unroll 1
-        for (int k = 0; k < n; ++k) {
-          const double pr = are[k * n + p], pi = aim[k * n + p];
-          const double qr = are[k * n + q], qi = aim[k * n + q];
-          const double qer = qr * er + qi * ei, qei = qi * er - qr * ei;  // a_kq e^{-i phi}
-          are[k * n + p] = c * pr - s * qer;
-          aim[k * n + p] = c * pi - s * qei;
-          are[k * n + q] = s * pr + c * qer;
-          aim[k * n + q] = s * pi + c * qei;
-          const double vpr = vre[k * n + p], vpi = vim[k * n + p];
-          const double vqr = vre[k * n + q], vqi = vim[k * n + q];
-          const double vqer = vqr * er + vqi * ei, vqei = vqi * er - vqr * ei;
-          vre[k * n + p] = c * vpr - s * vqer;
-          vim[k * n + p] = c * vpi - s * vqei;
-          vre[k * n + q] = s * vpr + c * vqer;
-          vim[k * n + q] = s * vpi + c * vqei;
-        }
-        //   row_p' = c row_p - s e^{i phi} row_q,  row_q' = s row_p + c e^{i phi} row_q
+      for (int i = 0; i < m / 2; ++i) {
+        if (qs[i] >= n || rot[i].skip) continue;
+        const int p = ps[i], q = qs[i];
 #pragma unroll 1
         for (int k = 0; k < n; ++k) {
-          const double pr = are[p * n + k], pi = aim[p * n + k];
-          const double qr = are[q * n + k], qi = aim[q * n + k];
-          const double qer = qr * er - qi * ei, qei = qi * er + qr * ei;  // a_qk e^{i phi}
-          are[p * n + k] = c * pr - s * qer;
-          aim[p * n + k] = c * pi - s * qei;
-          are[q * n + k] = s * pr + c * qer;
-          aim[q * n + k] = s * pi + c * qei;
+          jacobi_cols(rot[i], are, aim, k * n + p, k * n + q);
+          jacobi_cols(rot[i], vre, vim, k * n + p, k * n + q);
         }
+      }
+#pragma unroll 1
+      for (int i = 0; i < m / 2; ++i) {
+        if (qs[i] >= n || rot[i].skip) continue;
+        const int p = ps[i], q = qs[i];
+#pragma unroll 1
+        for (int k = 0; k < n; ++k) jacobi_rows(rot[i], are, aim, p * n + k, q * n + k);
         are[p * n + q] = aim[p * n + q] = are[q * n + p] = aim[q * n + p] = 0.0;
         aim[p * n + p] = aim[q * n + q] = 0.0;
       }
+    }
   }
   return sweep;
 }
 
-// hermitian_jacobi executed by one wavefront: the same rotations in the same order, lane k
-// updating row k's entries of columns p, q (and of V), then, after a wavefront fence, row p's
-// and q's entries of column k.  Every entry sees exactly the arithmetic of the serial version,
-// so the results are identical; the serial dependency is one rotation, not one element.
-// Call from all 64 lanes of one wavefront (n <= 8; other lanes idle in the updates).
+// hermitian_jacobi executed by one wavefront: lane (i, k) = (lane / n, lane % n) for the i-th pair of
+// a round applies that pair's column update to row k (of A and V), then, after a wavefront fence,
+// its row update to column k; the same rotations and arithmetic as the serial version, so the
+// results are identical.  Call from all 64 lanes of one wavefront (n <= 8).
 __device__ inline int hermitian_jacobi_wave(int n, double* are, double* aim, double* vre, double* vim,
                                             int max_sweeps = 30) {
-  const int k = static_cast<int>(threadIdx.x & 63);
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int k = lane % n, pair = lane / n;
   auto sync = [] {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   };
-  if (k < n)
+  if (lane < n)
     for (int j = 0; j < n; ++j) {
-      vre[k * n + j] = (k == j) ? 1.0 : 0.0;
-      vim[k * n + j] = 0.0;
+      vre[lane * n + j] = (lane == j) ? 1.0 : 0.0;
+      vim[lane * n + j] = 0.0;
     }
   sync();
+  const int m = n + (n & 1);
   int sweep = 0;
 #pragma unroll 1
   for (; sweep < max_sweeps; ++sweep) {
@@ -118,50 +168,27 @@ __device__ inline int hermitian_jacobi_wave(int n, double* are, double* aim, dou
     }
     if (off <= 1e-30 * diag || off == 0.0) break;
 #pragma unroll 1
-    for (int p = 0; p < n - 1; ++p)
-#pragma unroll 1
-      for (int q = p + 1; q < n; ++q) {
-        const double xr = are[p * n + q], xi = aim[p * n + q];
-        const double r = sqrt(xr * xr + xi * xi);
-        if (r == 0.0) continue;
-        const double er = xr / r, ei = xi / r;
-        const double tau = (are[q * n + q] - are[p * n + p]) / (2.0 * r);
-        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-        sync();  // every lane has read the pivot before it changes
-        if (k < n) {
-          const double pr = are[k * n + p], pi = aim[k * n + p];
-          const double qr = are[k * n + q], qi = aim[k * n + q];
-          const double qer = qr * er + qi * ei, qei = qi * er - qr * ei;
-          are[k * n + p] = c * pr - s * qer;
-          aim[k * n + p] = c * pi - s * qei;
-          are[k * n + q] = s * pr + c * qer;
-          aim[k * n + q] = s * pi + c * qei;
-          const double vpr = vre[k * n + p], vpi = vim[k * n + p];
-          const double vqr = vre[k * n + q], vqi = vim[k * n + q];
-          const double vqer = vqr * er + vqi * ei, vqei = vqi * er - vqr * ei;
-          vre[k * n + p] = c * vpr - s * vqer;
-          vim[k * n + p] = c * vpi - s * vqei;
-          vre[k * n + q] = s * vpr + c * vqer;
-          vim[k * n + q] = s * vpi + c * vqei;
-        }
-        sync();
-        if (k < n) {
-          const double pr = are[p * n + k], pi = aim[p * n + k];
-          const double qr = are[q * n + k], qi = aim[q * n + k];
-          const double qer = qr * er - qi * ei, qei = qi * er + qr * ei;
-          are[p * n + k] = c * pr - s * qer;
-          aim[p * n + k] = c * pi - s * qei;
-          are[q * n + k] = s * pr + c * qer;
-          aim[q * n + k] = s * pi + c * qei;
-        }
-        sync();
-        if (k == 0) {
-          are[p * n + q] = aim[p * n + q] = are[q * n + p] = aim[q * n + p] = 0.0;
-          aim[p * n + p] = aim[q * n + q] = 0.0;
-        }
-        sync();
+    for (int round = 0; round < m - 1; ++round) {
+      int p = 0, q = n;
+      if (pair < m / 2) jacobi_round_pair(n, round, pair, &p, &q);
+      const bool active = pair < m / 2 && q < n;
+      JacobiRotation rot;
+      rot.skip = true;
+      if (active) rot = jacobi_rotation(are[p * n + p], are[q * n + q], are[p * n + q], aim[p * n + q]);
+      sync();  // every lane has read its pivot before the matrix changes
+      if (active && !rot.skip) {
+        jacobi_cols(rot, are, aim, k * n + p, k * n + q);
+        jacobi_cols(rot, vre, vim, k * n + p, k * n + q);
       }
+      sync();
+      if (active && !rot.skip) jacobi_rows(rot, are, aim, p * n + k, q * n + k);
+      sync();
+      if (active && !rot.skip && k == 0) {
+        are[p * n + q] = aim[p * n + q] = are[q * n + p] = aim[q * n + p] = 0.0;
+        aim[p * n + p] = aim[q * n + q] = 0.0;
+      }
+      sync();
+    }
   }
   return sweep;
 }
