@@ -90,6 +90,13 @@ __host__ __device__ inline bool modulation(int order, int is_complex, Modulation
   }
 }
 
+// Bits of the two axes of a complex symbol interleaved: bit i of the real axis to bit 2 i, of the
+// imaginary axis to bit 2 i + 1 (an axis carries at most 4 bits); no loop over the bit count.
+__host__ __device__ __forceinline__ uint32_t interleave_axis_bits(uint32_t br, uint32_t bi) {
+  auto spread = [](uint32_t x) { return (x & 1u) | ((x & 2u) << 1) | ((x & 4u) << 2) | ((x & 8u) << 3); };
+  return spread(br) | (spread(bi) << 1);
+}
+
 // Hard bits of one symbol, bit i of the result = i-th bit of the symbol
 // (the order in which detect_bits emits them).
 __host__ __device__ inline uint32_t demap_symbol(const Modulation& m, double re, double im) {
@@ -102,13 +109,7 @@ __host__ __device__ inline uint32_t demap_symbol(const Modulation& m, double re,
   const double ri = m.pam == 2 ? im * m.pre : (im * m.pre) * m.scale;
   const uint32_t br = pam_decide(m.pam, rr, 1);
   const uint32_t bi = pam_decide(m.pam, ri, 1);
-  const int nb = m.bps / 2;
-  uint32_t out = 0;
-  for (int i = 0; i < nb; ++i) {
-    out |= ((br >> i) & 1u) << (2 * i);
-    out |= ((bi >> i) & 1u) << (2 * i + 1);
-  }
-  return out;
+  return interleave_axis_bits(br, bi);
 }
 
 // ---------------------------------------------------------------- float32 inputs
@@ -229,26 +230,16 @@ __host__ __device__ __forceinline__ uint32_t pam_decide_f32_reg(float x, const f
 // 256-QAM), boundaries in registers as for pam_decide_f32_reg.
 template <int PAM>
 __host__ __device__ __forceinline__ uint32_t demap_symbol_f32_reg(float re, float im, const float* ta, const float* tb) {
-  constexpr int nb = PAM == 2 ? 1 : PAM == 4 ? 2 : PAM == 8 ? 3 : 4;  // bits per axis
   const uint32_t br = pam_decide_f32_reg<PAM>(re, ta, tb);
   const uint32_t bi = pam_decide_f32_reg<PAM>(im, ta, tb);
-  uint32_t out = 0;
-#pragma unroll
-  for (int i = 0; i < nb; ++i) out |= (((br >> i) & 1u) << (2 * i)) | (((bi >> i) & 1u) << (2 * i + 1));
-  return out;
+  return interleave_axis_bits(br, bi);
 }
 
 // demap_symbol of a complex constellation for float32 components (bit-identical decisions).
 __host__ __device__ inline uint32_t demap_symbol_f32(const Modulation& m, const DemapBounds* d, float re, float im) {
   const uint32_t br = pam_decide_f32(m.pam, re, d);
   const uint32_t bi = pam_decide_f32(m.pam, im, d);
-  const int nb = m.bps / 2;
-  uint32_t out = 0;
-  for (int i = 0; i < nb; ++i) {
-    out |= ((br >> i) & 1u) << (2 * i);
-    out |= ((bi >> i) & 1u) << (2 * i + 1);
-  }
-  return out;
+  return interleave_axis_bits(br, bi);
 }
 
 // Label (MSB first) of a symbol's bits given bit i of `bits` = i-th bit.
