@@ -284,6 +284,95 @@ __device__ __forceinline__ void fft_pass_layout(float2* data, int count, int str
   else fft_pass_mixed<SIGN, LOG2R, MAXV, true, true>(data, count, stride, log2n, log2ns, tab, shift);
 }
 
+// Strength-reduced passes for N >= 512 in the padded layout of shift kFftPadShift, where every
+// pass's access pattern is fixed by its position: pass 0 reads the natural layout and writes
+// outputs r = 0..15 of an item at dst + r (dst a multiple of 16: one padding group, so the padded
+// offsets are pad(dst) + r); pass 1 reads at j + k nr (nr = N / 16, a multiple of 32: padded
+// pad(j) + k (nr + nr / 32)) and writes at dst + 16 r (the even r at pad(dst) + 16 r + r / 2, the
+// odd r at pad(dst + 16) + 16 (r - 1) + (r - 1) / 2); later passes read and write at multiples
+// of 32 apart (base plus a padded stride), the last one in the natural layout.  Each access is
+// then one address add or an immediate offset instead of the per-element padding arithmetic.
+enum { kFastInNatural = 0, kFastInPadded = 1 };
+enum { kFastOutUnit = 0, kFastOutHalf = 1, kFastOutPadded = 2, kFastOutNatural = 3 };
+
+template <int SIGN, int LOG2R, int MAXV, int IN, int OUT>
+__device__ __forceinline__ void fft_pass_fast(float2* data, int count, int stride, int log2n, int log2ns,
+                                              const float2* tab) {
+  constexpr int R = 1 << LOG2R;
+  constexpr int S = kFftPadShift;
+  constexpr int kItems = MAXV / R > 0 ? MAXV / R : 1;
+  const int log2nr = log2n - LOG2R;
+  const int nr = 1 << log2nr;
+  const int ns = 1 << log2ns;
+  const int total = count << log2nr;
+  const int in_step = IN == kFastInNatural ? nr : nr + (nr >> S);
+  const int out_step = OUT == kFastOutPadded ? ns + (ns >> S) : ns;
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  float2 v[kItems][R];
+  int dst[kItems];
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const int item = tid + it * blockDim.x;
+    if (item < total) {
+      const int j = item & (nr - 1);
+      const int pos = j & (ns - 1);
+      const float2* bp = data + (item >> log2nr) * stride + (IN == kFastInNatural ? j : fft_pad(j, S));
+#pragma unroll
+      for (int k = 0; k < R; ++k) v[it][k] = bp[k * in_step];
+      if (ns > 1) {
+        const float2* ptab = tab + (ns - 1) + pos;  // passes before this one hold ns - 1 entries
+#pragma unroll
+        for (int k = 1; k < R; ++k) v[it][k] = cmul(v[it][k], pass_twiddle<SIGN>(ptab, (k - 1) * ns));
+      }
+      if (R == 16) {
+        dft16<SIGN>(v[it]);
+      } else if (R == 4) {
+        dft4<SIGN>(v[it][0], v[it][1 % R], v[it][2 % R], v[it][3 % R]);
+      } else {
+        const float2 a0 = v[it][0], a1 = v[it][1 % R];
+        v[it][0] = cadd(a0, a1);
+        v[it][1 % R] = csub(a0, a1);
+      }
+      dst[it] = ((j - pos) << LOG2R) + pos;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const int item = tid + it * blockDim.x;
+    if (item < total) {
+      float2* row = data + (item >> log2nr) * stride;
+      const int d = dst[it];
+      if (OUT == kFastOutUnit) {
+        float2* b = row + fft_pad(d, S);
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[r] = v[it][r];
+      } else if (OUT == kFastOutHalf) {  // ns = 2^(S - 1)
+        float2* b0 = row + fft_pad(d, S);
+        float2* b1 = row + fft_pad(d + (1 << (S - 1)), S);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r & 1) b1[(r - 1) * (1 << (S - 1)) + (r - 1) / 2] = v[it][r];
+          else b0[r * (1 << (S - 1)) + r / 2] = v[it][r];
+        }
+      } else {
+        float2* b = row + (OUT == kFastOutPadded ? fft_pad(d, S) : d);
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[r * out_step] = v[it][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int SIGN, int LOG2R, int MAXV, int IN>
+__device__ __forceinline__ void fft_pass_fast_out(float2* data, int count, int stride, int log2n, int log2ns,
+                                                  const float2* tab, bool last) {
+  if (last) fft_pass_fast<SIGN, LOG2R, MAXV, IN, kFastOutNatural>(data, count, stride, log2n, log2ns, tab);
+  else fft_pass_fast<SIGN, LOG2R, MAXV, IN, kFastOutPadded>(data, count, stride, log2n, log2ns, tab);
+}
+
 // In-place complex FFT (sign SIGN, no scaling) of `count` vectors at data + v * stride in LDS,
 // natural order in and out, with the pass table `tab` (fill_fft_pass_table) and the padded
 // intermediate layout of pad shift `shift`.  Called by every thread; ends with a barrier.
@@ -294,9 +383,24 @@ __device__ __forceinline__ void lds_fft_padded(float2* data, int count, int stri
   static_assert(MAXV % 4 == 0, "MAXV must be a multiple of 4");
   const int group = max(1, (MAXV * static_cast<int>(blockDim.x)) >> log2n);
   const int passes = fft_num_passes(log2n);
+  const bool fast = shift == kFftPadShift && log2n >= 9;  // N >= 512: passes 0 and 1 are radix 16
   for (int g0 = 0; g0 < count; g0 += group) {
     const int cnt = min(group, count - g0);
     float2* base = data + g0 * stride;
+    if (fast) {
+      fft_pass_fast<SIGN, 4, MAXV, kFastInNatural, kFastOutUnit>(base, cnt, stride, log2n, 0, tab);
+      fft_pass_fast<SIGN, 4, MAXV, kFastInPadded, kFastOutHalf>(base, cnt, stride, log2n, 4, tab);
+      int log2ns = 8;
+      for (int p = 2; p < passes; ++p) {
+        const int lr = fft_pass_log2_radix(log2n, p);
+        const bool last = p == passes - 1;
+        if (lr == 4) fft_pass_fast_out<SIGN, 4, MAXV, kFastInPadded>(base, cnt, stride, log2n, log2ns, tab, last);
+        else if (lr == 2) fft_pass_fast_out<SIGN, 2, MAXV, kFastInPadded>(base, cnt, stride, log2n, log2ns, tab, last);
+        else fft_pass_fast_out<SIGN, 1, MAXV, kFastInPadded>(base, cnt, stride, log2n, log2ns, tab, last);
+        log2ns += lr;
+      }
+      continue;
+    }
     int log2ns = 0;
     for (int p = 0; p < passes; ++p) {
       const int lr = fft_pass_log2_radix(log2n, p);
