@@ -25,9 +25,9 @@ FP32_UNPACKED_PEAK_TFLOPS = 78.6  # one FP32 FMA per lane and cycle: the kernels
 HBM_PEAK_GBPS = 8000.0  # published MI355X HBM3E bandwidth
 # HBM bytes of one fused 4x4 drop, from rocprofv3 FETCH_SIZE / WRITE_SIZE over 45 056-drop launches
 # (FETCH x2.0 and WRITE x1.0, the factors a 256 MiB device copy reads in the same process):
-# (2084 x 2 + 1408) KiB / 45056; the algorithmic part is ~12 B (noise power, error count) plus the
+# (1990 x 2 + 1408) KiB / 45056; the algorithmic part is ~12 B (noise power, error count) plus the
 # link tables every drop reads from L2
-HBM_BYTES_PER_DROP = (2084.0 * 2.0 + 1408.0) * 1024.0 / 45056.0
+HBM_BYTES_PER_DROP = (1990.0 * 2.0 + 1408.0) * 1024.0 / 45056.0
 DROPS_PER_STEP = 11 * 4096  # one step: the campaign's 4096 drops at each of its 11 SNR points, one launch
 SEED = 20261202
 
