@@ -249,7 +249,7 @@ def uneven_leg(drops: int = 8192, reps: int = 5) -> dict:
             "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "flops_per_launch": flops,
                          "note": "the population's algorithmic flops over the graph's span"},
-            "workgroup_choice": tuning["choice"]}
+            "class_choice": tuning["choice"], "class_choice_fields": ["workgroup_size", "fft_values"]}
 
 
 def campaign_leg(num_samples: int = 2048) -> dict:

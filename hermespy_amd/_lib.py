@@ -16,7 +16,7 @@ import torch  # noqa: F401  (must precede the library load, see module docstring
 
 LIB_PATH = Path(__file__).resolve().parent / "_native" / "libhermespy_amd.so"
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 HPX_OK = 0
 HPX_ERR_INVALID = -1
@@ -54,7 +54,7 @@ class HpxLink(ctypes.Structure):
         ("dc_suppression", c_i32),
         ("bandwidth", c_dbl), ("doppler", c_dbl), ("los_doppler", c_dbl), ("gain", c_dbl),
         ("device_power", c_dbl),
-        ("workgroup_size", c_i32),
+        ("workgroup_size", c_i32), ("fft_values", c_i32),
     ]
 
 
@@ -94,7 +94,7 @@ class HpxUnevenClass(ctypes.Structure):
     """Mirror of ``hpx_uneven_class``."""
 
     _fields_ = [("first", c_i64), ("count", c_i64), ("num_tx", c_i32), ("threads", c_i32), ("pairs", c_i32),
-                ("lean", c_i32), ("lds_bytes", c_i64)]
+                ("lean", c_i32), ("lds_bytes", c_i64), ("fft_values", c_i32), ("reserved", c_i32)]
 
 
 class HpxMassiveLink(ctypes.Structure):

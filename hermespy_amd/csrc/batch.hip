@@ -41,13 +41,19 @@ namespace {
 
 constexpr int kBatchMaxThreads = 512;
 constexpr int kTailPerThread = 4;  // registers holding the previous word's tail
-// Complex values per thread in one FFT pass: 12 where the kernel runs in 128 registers
-// (512 threads, at most 4 antennas), which keeps the FFT's registers from spilling (5 rows of
-// 1024 points still form one group; measured 3 % faster per 4x4 1024-subcarrier drop than
-// 16, and 2.5 % faster again with the current channel loop); 16 where registers are ample.
+// Complex values per thread in one FFT pass (the rows a group of the transform holds is this
+// times the threads over N): by default 12 where the kernel runs in 128 registers (512 threads,
+// at most 4 antennas), which keeps the FFT's registers from spilling (5 rows of 1024 points
+// still form one group); 16 where registers are ample.  Heterogeneous batches may take the
+// alternative instead, a tuning choice per class: 16 where the default is 12 (fewer groups,
+// more registers), else 8 for 8 antennas (fewer registers, more groups) and 12 otherwise.
+constexpr int fft_values_for(int nt, int maxt) { return maxt >= 512 && nt <= 4 ? 12 : 16; }
+constexpr int alt_fft_values_for(int nt, int maxt) {
+  return fft_values_for(nt, maxt) == 12 ? 16 : (nt == 8 ? 8 : 12);
+}
 template <int NT, int MAXT>
 constexpr int fft_values() {
-  return MAXT >= 512 && NT <= 4 ? 12 : 16;
+  return fft_values_for(NT, MAXT);
 }
 
 struct BatchArgs {
@@ -569,16 +575,99 @@ batch_kernel(BatchArgs a) {
 
 // Heterogeneous batches: workgroup i simulates the drop plans[i] describes; its first_drop is
 // the drop's Philox index and its per-drop pointers already point at the drop's entries.
-template <int NT, int NR, int PAIRS, int MAXT, bool LEAN = false>
+template <int NT, int NR, int PAIRS, int MAXT, bool LEAN = false, int FFTV = fft_values<NT, MAXT>()>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu<NT, MAXT>())))
 batch_kernel_multi(const BatchArgs* __restrict__ plans) {
-  drop_body<NT, NR, PAIRS, fft_values<NT, MAXT>(), LEAN>(plans[blockIdx.x], 0);
+  drop_body<NT, NR, PAIRS, FFTV, LEAN>(plans[blockIdx.x], 0);
 }
+
+// Raises a kernel's dynamic LDS limit once per process and size (so that launches recorded
+// into a graph after a first eager run make no attribute calls).
+bool ensure_lds(const void* fn, int64_t bytes) {
+  static std::mutex mu;
+  static std::map<const void*, int64_t> granted;
+  if (bytes <= 64 * 1024) return true;
+  std::lock_guard<std::mutex> lock(mu);
+  int64_t& g = granted[fn];
+  if (g >= bytes) return true;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)) != hipSuccess)
+    return false;
+  g = bytes;
+  return true;
+}
+
+template <int NT, int MAXT, bool LEAN, int FFTV>
+int launch_multi_kernel(const BatchArgs* plans, const hpx_uneven_class& c, hipStream_t stream) {
+  const void* fn = nullptr;
+  switch (c.pairs) {
+    case 1: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 1, MAXT, LEAN, FFTV>); break;
+    case 2: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 2, MAXT, LEAN, FFTV>); break;
+    case 4: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 4, MAXT, LEAN, FFTV>); break;
+    default: set_error("hpx_uneven_run: unsupported samples per thread"); return HPX_ERR_UNSUPPORTED;
+  }
+  if (!ensure_lds(fn, c.lds_bytes)) {
+    set_error("hpx_uneven_run: %lld bytes of LDS per workgroup not available", static_cast<long long>(c.lds_bytes));
+    return HPX_ERR_UNSUPPORTED;
+  }
+  const BatchArgs* first = plans + c.first;
+  const dim3 grid(static_cast<unsigned>(c.count)), block(c.threads);
+  switch (c.pairs) {
+    case 1: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 1, MAXT, LEAN, FFTV>), grid, block, c.lds_bytes, stream, first); break;
+    case 2: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 2, MAXT, LEAN, FFTV>), grid, block, c.lds_bytes, stream, first); break;
+    default: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 4, MAXT, LEAN, FFTV>), grid, block, c.lds_bytes, stream, first); break;
+  }
+  return check_launch("hpx_uneven_run");
+}
+
+// One class launch with the FFT values per thread `fftv` (lean classes exist only for 8x8 links,
+// the one shape beyond one CU's LDS).
+template <int NT, int MAXT, int FFTV>
+int launch_multi_fftv(const BatchArgs* plans, const hpx_uneven_class& c, hipStream_t stream) {
+  if constexpr (NT == 8) {
+    if (c.lean) return launch_multi_kernel<NT, MAXT, true, FFTV>(plans, c, stream);
+  }
+  if (c.lean) {
+    set_error("hpx_uneven_run: lean class with %d antennas", NT);
+    return HPX_ERR_INVALID;
+  }
+  return launch_multi_kernel<NT, MAXT, false, FFTV>(plans, c, stream);
+}
+
+}  // namespace
+
+#ifdef HPX_BATCH_FFT_ALT
+// This translation unit holds the heterogeneous-batch kernels with the alternative FFT values per
+// thread; batch.hip launches them through this function.
+namespace hpx {
+int batch_launch_multi_alt(const void* plans_dev, const hpx_uneven_class& c, int maxt, hipStream_t s) {
+  const BatchArgs* p = static_cast<const BatchArgs*>(plans_dev);
+  const bool big = maxt > 256;
+  switch (c.num_tx) {
+    case 1: return big ? launch_multi_fftv<1, 512, alt_fft_values_for(1, 512)>(p, c, s)
+                       : launch_multi_fftv<1, 256, alt_fft_values_for(1, 256)>(p, c, s);
+    case 2: return big ? launch_multi_fftv<2, 512, alt_fft_values_for(2, 512)>(p, c, s)
+                       : launch_multi_fftv<2, 256, alt_fft_values_for(2, 256)>(p, c, s);
+    case 4: return big ? launch_multi_fftv<4, 512, alt_fft_values_for(4, 512)>(p, c, s)
+                       : launch_multi_fftv<4, 256, alt_fft_values_for(4, 256)>(p, c, s);
+    case 8: return big ? launch_multi_fftv<8, 512, alt_fft_values_for(8, 512)>(p, c, s)
+                       : launch_multi_fftv<8, 256, alt_fft_values_for(8, 256)>(p, c, s);
+  }
+  set_error("hpx_uneven_run: unsupported antenna count %d", c.num_tx);
+  return HPX_ERR_UNSUPPORTED;
+}
+}  // namespace hpx
+#else
+namespace hpx {
+int batch_launch_multi_alt(const void* plans_dev, const hpx_uneven_class& c, int maxt, hipStream_t s);
+}  // namespace hpx
+
+namespace {
 
 int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
 
 struct Plan {
   int threads = 0, pairs = 0;
+  int fftv = 0;  // FFT values per thread and pass
   int64_t lds = 0;
   bool lean = false;      // Taylor table and twiddles in device memory, ideal CSI per subcarrier
   int64_t ws_elems = 0;   // complex64 workspace elements of a lean drop
@@ -741,6 +830,16 @@ int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out, bool allow_lean 
   }
   out->threads = threads;
   out->pairs = pairs;
+  {
+    const int maxt = threads <= 256 ? 256 : 512;
+    const int def = fft_values_for(nt, maxt), alt = alt_fft_values_for(nt, maxt);
+    if (l->fft_values != 0 && l->fft_values != def && l->fft_values != alt) {
+      set_error("hpx_run_batch: fft_values must be 0, %d or %d for %d antennas at %d threads (got %d)", def, alt, nt,
+                threads, l->fft_values);
+      return HPX_ERR_INVALID;
+    }
+    out->fftv = l->fft_values != 0 ? l->fft_values : def;
+  }
 
   // LDS layout; the prologue tables (uniforms, SoS table) alias the signal buffer.  The FFT's
   // padded layout is taken where it keeps as many drops per CU as the plain one.
@@ -773,21 +872,6 @@ int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out, bool allow_lean 
   return HPX_OK;
 }
 
-// Raises a kernel's dynamic LDS limit once per process and size (so that launches recorded
-// into a graph after a first eager run make no attribute calls).
-bool ensure_lds(const void* fn, int64_t bytes) {
-  static std::mutex mu;
-  static std::map<const void*, int64_t> granted;
-  if (bytes <= 64 * 1024) return true;
-  std::lock_guard<std::mutex> lock(mu);
-  int64_t& g = granted[fn];
-  if (g >= bytes) return true;
-  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)) != hipSuccess)
-    return false;
-  g = bytes;
-  return true;
-}
-
 template <class K>
 int launch(K kernel, const Plan& plan, int64_t num_drops, hipStream_t stream) {
   if (!ensure_lds(reinterpret_cast<const void*>(kernel), plan.lds)) {
@@ -809,39 +893,15 @@ int launch_pairs(const Plan& plan, int64_t num_drops, hipStream_t stream) {
   return HPX_ERR_UNSUPPORTED;
 }
 
-template <int NT, int MAXT, bool LEAN>
-int launch_multi_kernel(const BatchArgs* plans, const hpx_uneven_class& c, hipStream_t stream) {
-  const void* fn = nullptr;
-  switch (c.pairs) {
-    case 1: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 1, MAXT, LEAN>); break;
-    case 2: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 2, MAXT, LEAN>); break;
-    case 4: fn = reinterpret_cast<const void*>(batch_kernel_multi<NT, NT, 4, MAXT, LEAN>); break;
-    default: set_error("hpx_uneven_run: unsupported samples per thread"); return HPX_ERR_UNSUPPORTED;
-  }
-  if (!ensure_lds(fn, c.lds_bytes)) {
-    set_error("hpx_uneven_run: %lld bytes of LDS per workgroup not available", static_cast<long long>(c.lds_bytes));
-    return HPX_ERR_UNSUPPORTED;
-  }
-  const BatchArgs* first = plans + c.first;
-  switch (c.pairs) {
-    case 1: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 1, MAXT, LEAN>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
-    case 2: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 2, MAXT, LEAN>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
-    default: hipLaunchKernelGGL((batch_kernel_multi<NT, NT, 4, MAXT, LEAN>), dim3(static_cast<unsigned>(c.count)), dim3(c.threads), c.lds_bytes, stream, first); break;
-  }
-  return check_launch("hpx_uneven_run");
-}
-
-// lean classes exist only for 8x8 links (the one shape beyond one CU's LDS)
+// A class launch: the default FFT values per thread here, the alternative through the
+// translation unit that holds those kernels.
 template <int NT, int MAXT>
 int launch_multi(const BatchArgs* plans, const hpx_uneven_class& c, hipStream_t stream) {
-  if constexpr (NT == 8) {
-    if (c.lean) return launch_multi_kernel<NT, MAXT, true>(plans, c, stream);
-  }
-  if (c.lean) {
-    set_error("hpx_uneven_run: lean class with %d antennas", NT);
-    return HPX_ERR_INVALID;
-  }
-  return launch_multi_kernel<NT, MAXT, false>(plans, c, stream);
+  if (c.fft_values == fft_values<NT, MAXT>()) return launch_multi_fftv<NT, MAXT, fft_values<NT, MAXT>()>(plans, c, stream);
+  if (c.fft_values == alt_fft_values_for(NT, MAXT)) return hpx::batch_launch_multi_alt(plans, c, MAXT, stream);
+  set_error("hpx_uneven_run: %d FFT values per thread not available for %d antennas at %d threads", c.fft_values, NT,
+            c.threads);
+  return HPX_ERR_INVALID;
 }
 
 }  // namespace
@@ -912,9 +972,11 @@ int uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_hos
     a.seed = seed;
     a.first_drop = drop_index_host[i];
   }
-  // kernel class = (antennas, threads, samples per thread, lean); heavier classes and drops first
+  // kernel class = (antennas, threads, samples per thread, lean, FFT values per thread); heavier
+  // classes and drops first
   auto key = [&](int64_t i) {
-    return std::make_tuple(links_host[i].num_tx, plans[i].threads, plans[i].pairs, plans[i].lean ? 1 : 0);
+    return std::make_tuple(links_host[i].num_tx, plans[i].threads, plans[i].pairs, plans[i].lean ? 1 : 0,
+                           plans[i].fftv);
   };
   auto cost = [&](int64_t i) {
     const hpx_link& l = links_host[i];
@@ -922,7 +984,7 @@ int uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_hos
   };
   std::vector<int64_t> idx(static_cast<size_t>(num_drops));
   for (int64_t i = 0; i < num_drops; ++i) idx[i] = i;
-  std::map<std::tuple<int, int, int, int>, double> class_cost;
+  std::map<std::tuple<int, int, int, int, int>, double> class_cost;
   for (int64_t i = 0; i < num_drops; ++i) class_cost[key(i)] = std::max(class_cost[key(i)], cost(i));
   std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
     const auto kx = key(x), ky = key(y);
@@ -952,6 +1014,8 @@ int uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_hos
       c.pairs = plans[i].pairs;
       c.lean = plans[i].lean ? 1 : 0;
       c.lds_bytes = 0;
+      c.fft_values = plans[i].fftv;
+      c.reserved = 0;
     }
     hpx_uneven_class& c = classes_host[nc - 1];
     c.count += 1;
@@ -1104,6 +1168,10 @@ extern "C" int hpx_run_batch_drops(const hpx_link* link_host, const hpx_frame* f
   Plan plan;
   const int st = make_plan(link_host, frame_host, &plan);
   if (st != HPX_OK) return st;
+  if (plan.fftv != fft_values_for(link_host->num_tx, plan.threads <= 256 ? 256 : 512)) {
+    set_error("hpx_run_batch: the alternative FFT values per thread apply to heterogeneous batches only");
+    return HPX_ERR_UNSUPPORTED;
+  }
   if (num_drops == 0) return HPX_OK;
   BatchArgs& a = plan.args;
   a.delays = delays;
@@ -1380,3 +1448,5 @@ extern "C" int hpx_run_batch_f64(const hpx_link* link_host, const int32_t* delay
                      g, reinterpret_cast<double2*>(equalized));
   return check_launch("hpx_run_batch_f64");
 }
+
+#endif  // HPX_BATCH_FFT_ALT

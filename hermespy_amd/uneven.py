@@ -71,12 +71,21 @@ def draw_population(num_drops: int, seed: int = 20261202, exclude=()) -> list:
     return out
 
 
-def _c_link(cfg: LinkConfig, workgroup_size: int = 0) -> _lib.HpxLink:
+def _c_link(cfg: LinkConfig, workgroup_size: int = 0, fft_values: int = 0) -> _lib.HpxLink:
     return _lib.HpxLink(cfg.num_rx, cfg.num_tx, cfg.num_subcarriers, cfg.words, cfg.prefix, cfg.modulation_order,
                         cfg.num_paths, cfg.num_sinusoids, cfg.max_delay, _lib.HPX_DETECT_MMSE,
                         int(bool(cfg.dc_suppression)), float(cfg.bandwidth), float(cfg.doppler),
                         float(cfg.doppler if cfg.los_doppler is None else cfg.los_doppler), float(cfg.gain),
-                        float(cfg.device_power), int(workgroup_size))
+                        float(cfg.device_power), int(workgroup_size), int(fft_values))
+
+
+def _choice(value) -> tuple:
+    """A class's launch configuration: a workgroup size, or (workgroup size, FFT values per thread);
+    0 stands for the library default."""
+
+    if isinstance(value, (tuple, list)):
+        return int(value[0]), int(value[1])
+    return int(value), 0
 
 
 def schedule_launches(classes, num_classes: int, cost, num_streams: int) -> tuple:
@@ -96,7 +105,7 @@ def schedule_launches(classes, num_classes: int, cost, num_streams: int) -> tupl
         for lo, hi in zip(cuts[:-1], cuts[1:]):
             if hi > lo:
                 pc = _lib.HpxUnevenClass(c.first + int(lo), int(hi - lo), c.num_tx, c.threads, c.pairs, c.lean,
-                                         c.lds_bytes)
+                                         c.lds_bytes, c.fft_values, 0)
                 pieces.append((cost[k] * (hi - lo) / c.count, pc))
     pieces.sort(key=lambda t: -t[0])
     load = np.zeros(num_streams)
@@ -169,11 +178,12 @@ class UnevenBatch:
         return (self.links[i].num_tx, self.links[i].num_subcarriers)
 
     def prepare(self, workgroup_sizes: dict) -> None:
-        """(Re)build the plans; workgroup_sizes maps (antennas, N) to a workgroup size (0: default)."""
+        """(Re)build the plans; workgroup_sizes maps (antennas, N) to a workgroup size or to a pair
+        (workgroup size, FFT values per thread), 0 meaning the library default."""
 
         n = len(self.links)
         self.workgroup_sizes = dict(workgroup_sizes)
-        links = (_lib.HpxLink * n)(*[_c_link(c, self.workgroup_sizes.get(self.class_key(i), 0))
+        links = (_lib.HpxLink * n)(*[_c_link(c, *_choice(self.workgroup_sizes.get(self.class_key(i), 0)))
                                      for i, c in enumerate(self.links)])
         idx = np.arange(self.first_drop, self.first_drop + n, dtype=np.uint32)
         plan_bytes = int(self.lib.hpx_uneven_plan_bytes())
@@ -267,14 +277,26 @@ def workgroup_candidates(n: int) -> list:
     return [t for t in (n // 2, n // 4, n // 8) if 64 <= t <= 512]
 
 
+def fft_value_candidates(num_tx: int, threads: int) -> list:
+    """The FFT values per thread and pass a class's kernel comes in (batch.hip's fft_values_for and
+    alt_fft_values_for): the default, 12 at 512 threads and at most 4 antennas, else 16; and the
+    alternative, 16 where the default is 12, 8 for 8 antennas, else 12."""
+
+    maxt = 256 if threads <= 256 else 512
+    default = 12 if maxt >= 512 and num_tx <= 4 else 16
+    alt = 16 if default == 12 else (8 if num_tx == 8 else 12)
+    return [default, alt]
+
+
 def tune(batch: UnevenBatch, reps: int = 3, verbose: bool = False) -> dict:
-    """Per (antennas, N) class, the workgroup size with the shortest launch; re-prepares the batch."""
+    """Per (antennas, N) class, the launch configuration (workgroup size, i.e. sample pairs per thread,
+    and FFT values per thread) with the shortest launch; re-prepares the batch."""
 
     keys = sorted({batch.class_key(i) for i in range(len(batch.links))})
     best = {}
     timings = {}
     for key in keys:
-        cands = workgroup_candidates(key[1])
+        cands = [(t, f) for t in workgroup_candidates(key[1]) for f in fft_value_candidates(key[0], t)]
         times = {}
         for t in cands:
             batch.prepare({**best, key: t})
@@ -298,5 +320,5 @@ def tune(batch: UnevenBatch, reps: int = 3, verbose: bool = False) -> dict:
                                  [best[batch.class_key(int(batch.order[batch.classes[k].first]))]]
                                  for k in range(batch.num_classes)])
     batch.schedule()
-    return {"choice": {f"{a}x{a}-{n}": t for (a, n), t in best.items()},
-            "ms": {f"{a}x{a}-{n}": {str(t): 1e3 * v for t, v in tt.items()} for (a, n), tt in timings.items()}}
+    return {"choice": {f"{a}x{a}-{n}": list(t) for (a, n), t in best.items()},
+            "ms": {f"{a}x{a}-{n}": {f"{t[0]}/{t[1]}": 1e3 * v for t, v in tt.items()} for (a, n), tt in timings.items()}}

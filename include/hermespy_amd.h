@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define HPX_ABI_VERSION 7
+#define HPX_ABI_VERSION 8
 
 typedef enum hpx_status {
   HPX_OK = 0,
@@ -239,6 +239,11 @@ typedef struct hpx_link {
   double gain;         /* linear channel gain */
   double device_power; /* transmit device power */
   int32_t workgroup_size; /* threads per drop: 0 = library default, else a power of two in [64, 512] */
+  int32_t fft_values;     /* complex values per thread and FFT pass: 0 = library default (12 at 512
+                             threads and at most 4 antennas, else 16); heterogeneous batches
+                             (hpx_uneven_prepare) also take the alternative, 16 where the default
+                             is 12, 8 for 8 antennas and 12 otherwise; the single-link entry
+                             points take only the default */
 } hpx_link;
 
 /* Device arrays of hpx_run_batch:
@@ -451,6 +456,8 @@ typedef struct hpx_uneven_class {
                                             twiddles in their workspace and compute the ideal
                                             CSI per subcarrier (8x8 at 2048 subcarriers) */
   int64_t lds_bytes;         /* dynamic LDS per workgroup (max over the class) */
+  int32_t fft_values;        /* complex values per thread and FFT pass of the class's kernel */
+  int32_t reserved;          /* 0 */
 } hpx_uneven_class;
 
 /* Complex64 elements of device workspace a drop of `link` needs in a heterogeneous batch: 0 when

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol(native_lib):
     for name in declared_functions():
         assert hasattr(native_lib, name), name
         assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
-    assert native_lib.hpx_abi_version() == _lib.ABI_VERSION == 7
+    assert native_lib.hpx_abi_version() == _lib.ABI_VERSION == 8
 
 
 def test_error_channel(native_lib):

@@ -127,6 +127,53 @@ def test_planner_rejects_bad_links(native_lib):
         _lib.check(st)
 
 
+def test_planner_splits_classes_by_fft_values(native_lib):
+    """The FFT values per thread are part of a drop's kernel class: drops of one shape that ask for
+    the default (0 or its value) and the alternative land in two classes that report their values;
+    a value the class has no kernel for is refused."""
+
+    from hermespy_amd import _lib
+    from hermespy_amd.uneven import MAX_CLASSES, DropShape, _c_link, fft_value_candidates
+
+    shapes = [DropShape(4, 2048, 9, 2.0, 100.0, 16, 10.0), DropShape(8, 1024, 7, 2.0, 300.0, 4, 20.0),
+              DropShape(2, 256, 5, 1.0, 50.0, 64, 5.0)]
+
+    def plan(requests):
+        n = len(requests)
+        c_links = (_lib.HpxLink * n)(*[_c_link(shapes[k].link(), 0, f) for k, f in requests])
+        tables = (_lib.HpxDropTables * n)()
+        for i in range(n):
+            tables[i].delays, tables[i].path_gains = 0x1000 + 256 * i, 0x100000 + 512 * i
+        idx = np.arange(n, dtype=np.uint32)
+        plans = np.zeros(n * int(native_lib.hpx_uneven_plan_bytes()), dtype=np.uint8)
+        order = np.empty(n, dtype=np.int64)
+        classes = (_lib.HpxUnevenClass * MAX_CLASSES)()
+        nc = _lib.c_int(0)
+        st = native_lib.hpx_uneven_prepare(c_links, tables, idx.ctypes.data, n, ctypes.c_uint64(1), 0x2000, 0x3000,
+                                           plans.ctypes.data, order.ctypes.data, classes, MAX_CLASSES,
+                                           ctypes.byref(nc))
+        return st, [classes[k] for k in range(nc.value)], order
+
+    requests = []
+    for k, shape in enumerate(shapes):
+        default, alt = fft_value_candidates(shape.antennas, min(512, shape.num_subcarriers // 2))
+        requests += [(k, 0), (k, alt), (k, default), (k, alt)]
+    st, classes, order = plan(requests)
+    _lib.check(st)
+    assert len(classes) == 2 * len(shapes)
+    for c in classes:
+        members = order[c.first:c.first + c.count]
+        ks = {requests[i][0] for i in members}
+        assert len(ks) == 1 and c.count == 2
+        k = ks.pop()
+        default, alt = fft_value_candidates(shapes[k].antennas, c.threads)
+        asked = {requests[i][1] or default for i in members}
+        assert asked == {c.fft_values} and c.fft_values in (default, alt) and c.reserved == 0
+    st, _, _ = plan([(0, 20)])
+    with pytest.raises(ValueError, match="fft_values"):
+        _lib.check(st)
+
+
 @pytest.mark.gpu
 def test_uneven_graph_replay_matches_single_link_launches(cuda):
     import torch
@@ -181,6 +228,41 @@ def test_uneven_autotuner_keeps_results(cuda):
     torch.cuda.synchronize()
     assert np.array_equal(ub.errors.cpu().numpy(), before)
     assert info["choice"]
+
+
+@pytest.mark.gpu
+def test_alternative_fft_values_keep_results(cuda):
+    """Every class on its alternative FFT values per thread (other row groups and per-pass register
+    counts, the same arithmetic per element): error counts and equalised symbols equal the default
+    kernels' bit for bit, lean 8x8 classes included."""
+
+    import torch
+
+    from hermespy_amd.uneven import DropShape, UnevenBatch, draw_population, fft_value_candidates
+
+    pop = draw_population(80, seed=31) + [DropShape(8, 2048, 9, 2.0, 300.0, 16, 12.0)]
+    print("alternative FFT values per thread vs default", flush=True)
+    ub = UnevenBatch(pop, seed=4, first_drop=70, export_equalized=True)
+    ub.run()
+    torch.cuda.synchronize()
+    errors = ub.errors.cpu().numpy().copy()
+    eq = [t.cpu().numpy().copy() for t in ub.equalized]
+    alt = {}
+    for shape in pop:
+        key = (shape.antennas, shape.num_subcarriers)
+        threads = min(512, shape.num_subcarriers // 2)
+        alt[key] = (threads, fft_value_candidates(shape.antennas, threads)[1])
+    ub.prepare(alt)
+    assert all(ub.classes[k].fft_values == alt[(ub.classes[k].num_tx, 2 * ub.classes[k].threads
+                                                * ub.classes[k].pairs)][1] for k in range(ub.num_classes))
+    ub.errors.fill_(-1)
+    for t in ub.equalized:
+        t.zero_()
+    ub.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(ub.errors.cpu().numpy(), errors)
+    for i, t in enumerate(ub.equalized):
+        assert np.array_equal(t.cpu().numpy(), eq[i]), i
 
 
 @pytest.mark.gpu
