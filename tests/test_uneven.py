@@ -377,7 +377,8 @@ def test_launch_schedule_covers_every_slot_once_and_balances():
     rng = np.random.default_rng(5)
     counts = rng.integers(1, 500, 20)
     firsts = np.concatenate([[0], np.cumsum(counts)[:-1]])
-    classes = [_lib.HpxUnevenClass(int(f), int(c), int(rng.choice([1, 2, 4, 8])), 256, 1, 0, 4096)
+    classes = [_lib.HpxUnevenClass(int(f), int(c), int(rng.choice([1, 2, 4, 8])), 256, 1, 0, 4096,
+                                   int(rng.choice([8, 12, 16])), 0)
                for f, c in zip(firsts, counts)]
     cost = counts * rng.uniform(0.5, 20.0, counts.size)
     lanes, load = schedule_launches(classes, len(classes), cost, 4)
@@ -388,6 +389,7 @@ def test_launch_schedule_covers_every_slot_once_and_balances():
             k = int(np.searchsorted(firsts, pc.first, side="right") - 1)
             assert pc.first + pc.count <= firsts[k] + counts[k]
             assert (pc.num_tx, pc.threads, pc.lds_bytes) == (classes[k].num_tx, 256, 4096)
+            assert (pc.pairs, pc.lean, pc.fft_values) == (classes[k].pairs, classes[k].lean, classes[k].fft_values)
             slots[pc.first:pc.first + pc.count] += 1
             biggest = max(biggest, cost[k] * pc.count / counts[k])
     assert np.all(slots == 1)
