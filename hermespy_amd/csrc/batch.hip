@@ -55,6 +55,9 @@ template <int NT, int MAXT>
 constexpr int fft_values() {
   return fft_values_for(NT, MAXT);
 }
+// Single-link kernels of at most 256 threads keep only the spatial realisation uniforms in a table
+// (heterogeneous-batch kernels always do).
+constexpr bool compact_uniforms(int maxt) { return maxt <= 256; }
 
 struct BatchArgs {
   // link
@@ -114,7 +117,12 @@ __device__ __forceinline__ uint32_t symbol_bits(const uint32_t* bw, int bit, int
 // One drop, simulated by the whole workgroup.  local_drop indexes the per-drop arrays
 // (noise power, Doppler, errors, exports); the Philox drop index is drop_index[local_drop]
 // or first_drop + local_drop.
-template <int NT, int NR, int PAIRS, int FFTV, bool LEAN = false>
+// COMPACT: only the 100 spatial uniforms get a table, which takes up to ~8 KB off the LDS of links
+// of at most 512 subcarriers, whose prologue tables bound it.  The heterogeneous-batch kernels and
+// the single-link kernels of at most 256 threads take it; the 512-thread single-link kernels keep
+// the full table and their code as it is (the compact form measured 5 registers more and 2.4 %
+// slower on the 4x4 1024-subcarrier link).
+template <int NT, int NR, int PAIRS, int FFTV, bool LEAN = false, bool COMPACT = false>
 __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t local_drop) {
   extern __shared__ __align__(16) unsigned char lds[];
   float2* xbuf = reinterpret_cast<float2*>(lds + a.off_x);
@@ -149,14 +157,67 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
   // ---------------------------------------------------------------- prologue
   // realisation uniforms Phi(z), z ~ N(0,1) (consistent.py:432,485), or z of a recorded
   // StaticConsistentRealization (consistent.py:244-260)
-  if (a.normals) {
-    const double* z = a.normals + local_drop * a.nnorm;
-    for (int c = tid; c < a.nnorm; c += nthreads) uni[c] = normal_cdf(z[c]);
+  if constexpr (COMPACT) {
+    // Uniforms 0..99 (the spatial phases, read by several S entries) go to uni; every later one is
+    // read once and goes straight into its field of the SoS table (fading.py:326-342,
+    // angles/phases :492-499; rows [path][NLOS terms, then LOS] of {rate, phase, gain}): per
+    // path, LOS angles, NLOS angles, LOS phases, NLOS phases, in the generator's variable order.
+    const int ns = a.sinusoids, nlos = L * ns;
+    const double inv_fs = 1.0 / a.bandwidth;
+    const double dop = a.doppler_drop ? a.doppler_drop[2 * local_drop] : a.doppler;
+    const double los_dop = a.doppler_drop ? a.doppler_drop[2 * local_drop + 1] : a.los_doppler;
+    auto put = [&](int i, double u) {
+      if (i < 100) {
+        uni[i] = u;
+        return;
+      }
+      int j = i - 100;
+      if (j < L) {  // LOS angle of path j
+        const double ang = 2.0 * M_PI * u;
+        sos[3 * (j * a.terms + ns)] = los_dop * cos(ang) * inv_fs;
+        return;
+      }
+      j -= L;
+      if (j < nlos) {  // NLOS angle of sinusoid k of path p
+        const int p = j / ns, k = j - p * ns;
+        const double ang = -M_PI + 2.0 * M_PI * u;
+        sos[3 * (p * a.terms + k)] = dop * cos((2.0 * M_PI * (k + 1) + ang) / ns) * inv_fs;
+        return;
+      }
+      j -= nlos;
+      if (j < L) {  // LOS phase of path j
+        sos[3 * (j * a.terms + ns) + 1] = -M_PI + 2.0 * M_PI * u;
+        return;
+      }
+      j -= L;
+      const int p = j / ns, k = j - p * ns;  // NLOS phase
+      sos[3 * (p * a.terms + k) + 1] = -M_PI + 2.0 * M_PI * u;
+    };
+    if (a.normals) {
+      const double* z = a.normals + local_drop * a.nnorm;
+      for (int c = tid; c < a.nnorm; c += nthreads) put(c, normal_cdf(z[c]));
+    } else {
+      for (int c = tid; 2 * c < a.nnorm; c += nthreads) {
+        const double2 z = normal_pair_f64(stream_call(a.seed, drop, HPX_STREAM_REALIZATION, static_cast<uint32_t>(c)));
+        put(2 * c, normal_cdf(z.x));
+        if (2 * c + 1 < a.nnorm) put(2 * c + 1, normal_cdf(z.y));
+      }
+    }
+    for (int e = tid; e < L * a.terms; e += nthreads) {
+      const int p = e / a.terms, k = e - p * a.terms;
+      sos[3 * e + 2] = a.path_gains[k < ns ? L + p : p];
+    }
   } else {
-    for (int c = tid; 2 * c < a.nnorm; c += nthreads) {
-      const double2 z = normal_pair_f64(stream_call(a.seed, drop, HPX_STREAM_REALIZATION, static_cast<uint32_t>(c)));
-      uni[2 * c] = normal_cdf(z.x);
-      if (2 * c + 1 < a.nnorm) uni[2 * c + 1] = normal_cdf(z.y);
+    // all uniforms in uni; the SoS table is formed from them below
+    if (a.normals) {
+      const double* z = a.normals + local_drop * a.nnorm;
+      for (int c = tid; c < a.nnorm; c += nthreads) uni[c] = normal_cdf(z[c]);
+    } else {
+      for (int c = tid; 2 * c < a.nnorm; c += nthreads) {
+        const double2 z = normal_pair_f64(stream_call(a.seed, drop, HPX_STREAM_REALIZATION, static_cast<uint32_t>(c)));
+        uni[2 * c] = normal_cdf(z.x);
+        if (2 * c + 1 < a.nnorm) uni[2 * c + 1] = normal_cdf(z.y);
+      }
     }
   }
   if (tid == 0) red[0] = 0;
@@ -199,8 +260,11 @@ __device__ __forceinline__ void drop_body(const BatchArgs& a, const int64_t loca
     s64[2 * e] = sr;
     s64[2 * e + 1] = si;
   }
-  // SoS table (fading.py:326-342, angles/phases :492-499): NLOS terms, then LOS
-  {
+  if constexpr (COMPACT) {
+    // the word means' Dirichlet weight depends on the term only, not on the word
+    for (int e = tid; e < L * a.terms; e += nthreads) amp_dir[e] = sos[3 * e + 2] * dirichlet_factor(sos[3 * e], n);
+  } else {
+    // SoS table (fading.py:326-342, angles/phases :492-499): NLOS terms, then LOS
     const int ns = a.sinusoids;
     const int o_los_ang = 100, o_nlos_ang = 100 + L, o_los_ph = 100 + L + L * ns, o_nlos_ph = 100 + 2 * L + L * ns;
     const double inv_fs = 1.0 / a.bandwidth;
@@ -570,7 +634,7 @@ constexpr int batch_waves_per_eu() {
 template <int NT, int NR, int PAIRS, int MAXT>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu<NT, MAXT>())))
 batch_kernel(BatchArgs a) {
-  drop_body<NT, NR, PAIRS, fft_values<NT, MAXT>()>(a, blockIdx.x);
+  drop_body<NT, NR, PAIRS, fft_values<NT, MAXT>(), false, compact_uniforms(MAXT)>(a, blockIdx.x);
 }
 
 // Heterogeneous batches: workgroup i simulates the drop plans[i] describes; its first_drop is
@@ -578,7 +642,7 @@ batch_kernel(BatchArgs a) {
 template <int NT, int NR, int PAIRS, int MAXT, bool LEAN = false, int FFTV = fft_values<NT, MAXT>()>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu<NT, MAXT>())))
 batch_kernel_multi(const BatchArgs* __restrict__ plans) {
-  drop_body<NT, NR, PAIRS, FFTV, LEAN>(plans[blockIdx.x], 0);
+  drop_body<NT, NR, PAIRS, FFTV, LEAN, true>(plans[blockIdx.x], 0);
 }
 
 // Raises a kernel's dynamic LDS limit once per process and size (so that launches recorded
@@ -679,7 +743,7 @@ constexpr int64_t kMaxLds = 160 * 1024;
 // LDS layout of a plan; a lean layout leaves out the Taylor table, the FFT pass table and the
 // CSI row.  pad: rows long enough for the FFT's padded intermediate layout (N + N / 8 after the
 // first data sample).
-int64_t layout_lds(BatchArgs& a, int nr, int nt, bool lean, bool pad) {
+int64_t layout_lds(BatchArgs& a, int nr, int nt, bool lean, bool pad, bool compact) {
   const int rows = std::max(nr, nt) + (lean ? 0 : 1);  // signal rows (+ the CSI row)
   a.pad_shift = pad ? kFftPadShift : kFftNoPad;
   a.xs = a.x0 + a.n + (pad ? (a.n >> kFftPadShift) : 0);
@@ -687,7 +751,7 @@ int64_t layout_lds(BatchArgs& a, int nr, int nt, bool lean, bool pad) {
   int64_t o = 0;
   a.off_x = 0;
   const int64_t xbytes = static_cast<int64_t>(rows) * a.xs * sizeof(float2);
-  const int64_t uni_bytes = align16(static_cast<int64_t>(a.nnorm) * sizeof(double));
+  const int64_t uni_bytes = align16(static_cast<int64_t>(compact ? std::min(a.nnorm, 100) : a.nnorm) * sizeof(double));
   const int64_t sos_bytes = align16(static_cast<int64_t>(a.paths) * a.terms * 3 * sizeof(double));
   // S (float64), then the detector's outputs and Jacobi workspace
   const int64_t s64_bytes =
@@ -849,8 +913,10 @@ int make_plan(const hpx_link* l, const hpx_frame* f, Plan* out, bool allow_lean 
   };
   auto best_layout = [&](bool lean) {
     BatchArgs padded = a;
-    const int64_t op = layout_lds(padded, nr, nt, lean, true);
-    const int64_t on = layout_lds(a, nr, nt, lean, false);
+    // the kernel these threads launch: heterogeneous batches (allow_lean) always take the compact form
+    const bool compact = allow_lean || compact_uniforms(threads <= 256 ? 256 : 512);
+    const int64_t op = layout_lds(padded, nr, nt, lean, true, compact);
+    const int64_t on = layout_lds(a, nr, nt, lean, false, compact);
     if (per_cu(op) >= per_cu(on) && op <= kMaxLds) {
       a = padded;
       return op;
