@@ -1038,11 +1038,12 @@ int uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_hos
     a.seed = seed;
     a.first_drop = drop_index_host[i];
   }
-  // kernel class = (antennas, threads, samples per thread, lean, FFT values per thread); heavier
-  // classes and drops first
+  // kernel class = (antennas, threads, samples per thread, lean, FFT values per thread), split by
+  // subcarrier count so that a class holds one link shape (which a tuner times per shape; with
+  // the default workgroup sizes the kernel already determines it); heavier classes and drops first
   auto key = [&](int64_t i) {
     return std::make_tuple(links_host[i].num_tx, plans[i].threads, plans[i].pairs, plans[i].lean ? 1 : 0,
-                           plans[i].fftv);
+                           plans[i].fftv, links_host[i].num_subcarriers);
   };
   auto cost = [&](int64_t i) {
     const hpx_link& l = links_host[i];
@@ -1050,7 +1051,7 @@ int uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_hos
   };
   std::vector<int64_t> idx(static_cast<size_t>(num_drops));
   for (int64_t i = 0; i < num_drops; ++i) idx[i] = i;
-  std::map<std::tuple<int, int, int, int, int>, double> class_cost;
+  std::map<std::tuple<int, int, int, int, int, int>, double> class_cost;
   for (int64_t i = 0; i < num_drops; ++i) class_cost[key(i)] = std::max(class_cost[key(i)], cost(i));
   std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
     const auto kx = key(x), ky = key(y);

@@ -272,9 +272,11 @@ class UnevenBatch:
 
 
 def workgroup_candidates(n: int) -> list:
-    """Workgroup sizes with 1, 2 or 4 sample pairs per thread, 64..512 threads."""
+    """Workgroup sizes of 64..512 threads with 1, 2 or 4 sample pairs per thread, and N threads
+    (half of them idle in the sample-pair loops, all of them in the prologue, the transforms and
+    detection), which shortens the drops of the smallest links."""
 
-    return [t for t in (n // 2, n // 4, n // 8) if 64 <= t <= 512]
+    return [t for t in (n, n // 2, n // 4, n // 8) if 64 <= t <= 512]
 
 
 def fft_value_candidates(num_tx: int, threads: int) -> list:

@@ -471,9 +471,9 @@ int64_t hpx_uneven_plan_bytes(void);
 /* Builds the plans on the host (no device work): drop i has link links_host[i], tables
  * tables_host[i], Philox drop index drop_index_host[i], noise power noise_power[i] and error
  * count errors[i] (device arrays [num_drops]).  plans_host (num_drops plan bytes) receives the
- * plans grouped by class, heavier classes and drops first; order_host[k] = the drop at slot k;
- * classes_host (capacity max_classes) the classes.  Copy plans_host to the device for
- * hpx_uneven_run. */
+ * plans grouped by class (one kernel and one subcarrier count), heavier classes and drops first;
+ * order_host[k] = the drop at slot k; classes_host (capacity max_classes) the classes.  Copy
+ * plans_host to the device for hpx_uneven_run. */
 int hpx_uneven_prepare(const hpx_link* links_host, const hpx_drop_tables* tables_host, const uint32_t* drop_index_host,
                        int64_t num_drops, uint64_t seed, const double* noise_power, int32_t* errors, void* plans_host,
                        int64_t* order_host, hpx_uneven_class* classes_host, int max_classes, int* num_classes_host);
