@@ -132,6 +132,21 @@ def test_planner_rejects_bad_links(native_lib):
         _lib.check(st)
 
 
+def test_tuner_candidates():
+    """Launch configurations the heterogeneous tuner tries: N, N/2, N/4 and N/8 threads within
+    64..512, and the two FFT values per thread a class's kernels come in (default first)."""
+
+    from hermespy_amd.uneven import fft_value_candidates, workgroup_candidates
+
+    assert workgroup_candidates(128) == [128, 64]
+    assert workgroup_candidates(512) == [512, 256, 128, 64]
+    assert workgroup_candidates(1024) == [512, 256, 128]
+    assert workgroup_candidates(2048) == [512, 256]
+    assert fft_value_candidates(4, 512) == [12, 16] and fft_value_candidates(1, 512) == [12, 16]
+    assert fft_value_candidates(8, 512) == [16, 8] and fft_value_candidates(8, 64) == [16, 8]
+    assert fft_value_candidates(2, 256) == [16, 12] and fft_value_candidates(4, 128) == [16, 12]
+
+
 def test_planner_splits_classes_by_fft_values(native_lib):
     """The FFT values per thread are part of a drop's kernel class: drops of one shape that ask for
     the default (0 or its value) and the alternative land in two classes that report their values;
